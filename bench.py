@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the D3Q19 lid-driven-cavity collide-stream step on one MI355X.
+
+Workload: the 512^3 cavity (nu = 0.0089, lid u = 0.05), FP32 populations in
+the CSoA layout with stride 512^3 (planar populations), the row kernel.  A
+"step" is one collide-stream iteration of the whole lattice.  The timed region
+holds exactly --steps iterations enqueued back to back on the library's stream,
+bracketed by device synchronisation; the lattice is resident in HBM and already
+initialised before it starts.
+
+Printed: one JSON line with MLUPS (the reference's definition,
+(dim-2)^3 * iterations / time, lbmcl.hpp:601-620), the HBM roofline of the
+compute kernel (152 B per lattice update in FP32; average kernel duration from
+HIP events recorded on the kernel's stream around every launch of the timed
+region), the same figures for the 16-bit shifted-storage forms and FP64, a copy
+kernel's sustained bandwidth for context, and the CPU oracle timed on a bounded
+sample of the same workload (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {("f32", "native"): 152, ("f64", "native"): 304, ("f32", "fp16"): 76, ("f32", "bf16"): 76}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True):
+    from lbmcl_amd import capi
+    stride = stride or dim ** 3
+    with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=True,
+                      xcd_remap=xcd_remap) as g:
+        g.initialize()
+        it = 1
+        if warmup:
+            g.compute_range(it, warmup, 0)
+            it += warmup
+        g.finish()
+        g.clear_events()
+        t0 = time.perf_counter()
+        g.compute_range(it, steps, 0)
+        g.finish()
+        t1 = time.perf_counter()
+        kernel_ms = g.kernels_time_ms()
+        events = g.events()
+        total_ms = g.total_time_ms()
+    wet = (dim - 2) ** 3
+    wall_s = t1 - t0
+    avg_kernel_s = kernel_ms / 1e3 / steps
+    b = BYTES_PER_UPDATE[(precision, storage)]
+    achieved = wet * b / avg_kernel_s / 1e9
+    return {
+        "dim": dim, "precision": precision, "storage": storage, "stride": stride, "steps": steps,
+        "wall_s": wall_s, "ms_per_step": wall_s * 1e3 / steps,
+        "mlups": wet * steps / wall_s / 1e6,
+        "kernel_mlups": wet * steps / (kernel_ms / 1e3) / 1e6,
+        "avg_kernel_ms": kernel_ms / steps,
+        "event_span_ms": total_ms,
+        "n_events": len(events),
+        "bytes_per_update": b,
+        "achieved_gbs": achieved,
+        "frac": achieved / HBM_PEAK_GBS,
+    }
+
+
+def copy_bandwidth(nbytes=4 << 30, reps=5):
+    path = os.path.join(ROOT, "lbmcl_amd", "libmembw.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.membw_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ms = (ctypes.c_double * reps)()
+    if L.membw_copy(0, nbytes, reps, ms) != 0:
+        return None
+    best = min(ms[1:]) if reps > 1 else ms[0]
+    return {"bytes": nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
+
+
+def cpu_baseline(dim, steps, threads):
+    """The CPU oracle (reference push form, OpenMP) on `steps` iterations of the same lattice."""
+    import oracle as O
+    O.build()
+    sim = O.PushSim(dim, stride=dim ** 3, precision="f32", threads=threads)
+    sim.initialize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(update_macro=False)
+    dt = time.perf_counter() - t0
+    return {
+        "value": (dim - 2) ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+        "sample": f"oracle push form, {dim}^3 FP32 stride {dim}^3, {steps} iterations, {threads} OpenMP threads, "
+                  f"{dt:.1f} s",
+    }
+
+
+def traffic_from_profiles(dim, precision, storage):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, if committed."""
+    path = os.path.join(ROOT, "profiles", f"collide_stream_row_{dim}_{precision}_{storage}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", fh.read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dim", type=int, default=512)
+    ap.add_argument("--stride", type=int, default=0, help="CSoA stride (default dim^3)")
+    ap.add_argument("--variant-steps", type=int, default=300, help="steps of the FP16/BF16/FP64 runs (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="iterations of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-remap", action="store_true", help="disable the XCD-aware block order")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 row")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+    dim = args.dim
+    stride = args.stride or dim ** 3
+
+    log(f"[bench] main run: {dim}^3 f32 native stride {stride}, warmup {args.warmup}, steps {args.steps}")
+    main_run = run_lattice(dim, args.steps, args.warmup, "f32", "native", stride, args.kernel, not args.no_remap)
+    log(f"[bench] main: {main_run['mlups']:.0f} MLUPS, kernel {main_run['avg_kernel_ms']:.3f} ms, "
+        f"{main_run['achieved_gbs']:.0f} GB/s")
+
+    variants = {}
+    if args.variant_steps > 0:
+        for prec, st in (("f32", "fp16"), ("f32", "bf16"), ("f64", "native")):
+            log(f"[bench] variant {prec} {st}")
+            r = run_lattice(dim, args.variant_steps, max(5, args.warmup // 4), prec, st, stride, args.kernel,
+                            not args.no_remap)
+            variants[f"{prec}_{st}"] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs",
+                                                         "frac", "bytes_per_update", "steps")}
+            log(f"[bench] {prec} {st}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
+        variants["fp16_speedup_vs_f32"] = variants["f32_fp16"]["mlups"] / main_run["mlups"]
+
+    log("[bench] copy bandwidth")
+    copy = copy_bandwidth()
+    cpu = None
+    if args.cpu_steps > 0 and rank == 0:
+        log(f"[bench] cpu baseline: {args.cpu_steps} oracle iterations at {dim}^3")
+        cpu = cpu_baseline(dim, args.cpu_steps, args.cpu_threads)
+
+    wet = (dim - 2) ** 3
+    out = {
+        "metric": "MLUPS (D3Q19 lid-driven cavity, 512^3, FP32)",
+        "value": round(main_run["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (deterministic lid-driven cavity initial state; no dataset)",
+        "config": {"workload": f"D3Q19 BGK lid-driven cavity {dim}^3, nu=0.0089, lid u=0.05, FP32 populations, "
+                               f"CSoA stride {stride}, pull-form row kernel",
+                   "dim": dim, "stride": stride, "wet_cells": wet, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_run["frac"], 4),
+                     "traffic": traffic_from_profiles(dim, "f32", "native"),
+                     "kernel": "lbm_collide_stream_row<float, native>",
+                     "avg_kernel_ms": round(main_run["avg_kernel_ms"], 4),
+                     "bytes_per_launch": wet * 152},
+        "cpu_baseline": cpu,
+        "kernel_mlups": round(main_run["kernel_mlups"], 1),
+        "variants": variants,
+        "copy_bandwidth": copy,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,172 @@
+/*
+ * lbm_hip.h -- C ABI of liblbm_hip.so, the MI355X (gfx950) device layer of the
+ * D3Q19 lid-driven-cavity solver.
+ *
+ * It replaces the OpenCL device layer of LBMCL: platform/device selection,
+ * context and profiling queue (libs/CLUtil.hpp:119-199), the JIT build of
+ * kernels.cl with -D options (lbmcl.hpp:131-156, CLUtil.hpp:201-229), the five
+ * cl::Buffer objects (lbmcl.hpp:403-417), the `initialize` and `compute`
+ * kernels (kernels.cl:277-426) with their ping-pong argument binding
+ * (lbmcl.hpp:420-467, 490-521), the blocking buffer reads behind the dumps
+ * (lbmcl.hpp:159-334) and the event timing (lbmcl.hpp:545-593,
+ * CLUtil.hpp:231-243).
+ *
+ * Conventions: every entry point returns an int status (LBM_OK == 0); no C++
+ * exception crosses the ABI; the text of the last error of the calling thread
+ * is available from lbm_last_error().  The library owns device memory, the
+ * caller owns host buffers.  One context per host thread; each context owns one
+ * HIP stream and every enqueue goes to it in order.
+ *
+ * Device layout of the populations: the CSoA layout of the reference
+ * (kernels.cl:64), slot(c, q) = ((c / stride) * 19 + q) * stride + c % stride
+ * for cell c = x + y*dim + z*dim^2, with 64-bit slot arithmetic (the reference's
+ * int arithmetic overflows at dim = 512).  The device buffers hold the
+ * populations in pull ("gather") form -- the post-collision value at its source
+ * cell -- and lbm_read_f() returns them in the reference's push view, so the
+ * values a caller sees are the reference's.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LBM_OK 0
+#define LBM_ERR_INVALID (-1)   /* bad argument or configuration */
+#define LBM_ERR_DEVICE (-2)    /* no such device */
+#define LBM_ERR_MEMORY (-3)    /* device allocation failed */
+#define LBM_ERR_RUNTIME (-4)   /* HIP runtime error */
+#define LBM_ERR_STATE (-5)     /* call out of order (e.g. compute before initialize) */
+
+/* arithmetic precision (lbm_options.hpp:77 use_double, lbmcl.hpp:144-149) */
+#define LBM_F32 0
+#define LBM_F64 1
+
+/* population storage; the shifted forms store (f_i - w_i) * scale in 16 bits
+ * with FP32 arithmetic and have no counterpart in the reference */
+#define LBM_STORE_NATIVE 0
+#define LBM_STORE_FP16_SHIFT 1
+#define LBM_STORE_BF16_SHIFT 2
+
+/* kernel selection (LBM_KERNEL_AUTO picks the row kernel when dim >= 64) */
+#define LBM_KERNEL_AUTO 0
+#define LBM_KERNEL_GENERIC 1
+#define LBM_KERNEL_ROW 2
+
+/* event kinds, named like the reference's events (lbmcl.hpp:26-31) */
+#define LBM_EVENT_INITIALIZE 0
+#define LBM_EVENT_COMPUTE 1
+#define LBM_EVENT_READ_MAP 2
+#define LBM_EVENT_READ_F 3
+#define LBM_EVENT_READ_RHO 4
+#define LBM_EVENT_READ_U 5
+#define LBM_EVENT_DIAGNOSTICS 6
+
+typedef struct lbm_config {
+    int32_t dim;         /* lattice cube dimension, a power of two (lbmcl.hpp:366-369) */
+    int32_t precision;   /* LBM_F32 or LBM_F64 */
+    int32_t storage;     /* LBM_STORE_*; shifted storage requires LBM_F32 */
+    int32_t device;      /* HIP device ordinal (replaces -P/-D, CLUtil.hpp:119-178) */
+    int64_t stride;      /* CSoA stride, a power of two in [1, dim^3] (lbmcl.hpp:384-387) */
+    double viscosity;    /* kinematic viscosity nu; tau = 3 nu + 1/2 (kernels.cl:61-62) */
+    double velocity;     /* lid x-velocity U (kernels.cl:57) */
+    int32_t lws[3];      /* work-group size of the generic kernel (product <= 1024); 0 = default */
+    int32_t optimize;    /* -o: reserved, the kernels always compute without contraction */
+    int32_t kernel;      /* LBM_KERNEL_* */
+    int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
+    int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernel) */
+} lbm_config;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Number of HIP devices (CLUtil.hpp:145-178 lists devices). */
+int lbm_get_device_count(int32_t* count);
+
+/* Device name (replaces CL_DEVICE_NAME, lbmcl.hpp:626, 651). */
+int lbm_get_device_name(int32_t device, char* name, size_t len);
+
+/* Create a context: select the device, create the stream, allocate the two
+ * population lattices, rho, u and map (lbmcl.hpp:392-417).  viscosity and
+ * velocity are rounded the way the reference's -D build options round them:
+ * the value is converted to the arithmetic type, printed with 6 significant
+ * digits and parsed back as a literal of that type (lbmcl.hpp:140-146). */
+int lbm_create(const lbm_config* config, lbm_ctx** ctx);
+
+/* Release the context and all its device memory (lbmcl.hpp:672-678). */
+int lbm_destroy(lbm_ctx* ctx);
+
+/* Enqueue the `initialize` kernel (kernels.cl:277-318, lbmcl.hpp:493-498):
+ * map, rho = 1, u = (U,0,0) on FRONT cells and 0 elsewhere (NaN outside
+ * FLUID/MOVING cells), equilibrium populations in both lattices. */
+int lbm_initialize(lbm_ctx* ctx);
+
+/* Enqueue iteration `iteration` (>= 1) of the `compute` kernel
+ * (kernels.cl:321-426, lbmcl.hpp:505-511): odd iterations read lattice 0 and
+ * write lattice 1, even iterations the reverse (lbmcl.hpp:437, 456-457).
+ * update_macro stores rho and u of FLUID/MOVING cells (kernels.cl:383-388).
+ * diagnostics != 0 also reduces the mass sum(rho) and kinetic energy
+ * sum(rho |u|^2 / 2) over FLUID/MOVING cells in FP64; read them with
+ * lbm_read_diagnostics (no counterpart in the reference). */
+int lbm_compute(lbm_ctx* ctx, int64_t iteration, int32_t update_macro, int32_t diagnostics);
+
+/* Enqueue iterations first .. first+count-1; update_macro is set on those
+ * divisible by `every` when every > 0 (lbmcl.hpp:436). */
+int lbm_compute_range(lbm_ctx* ctx, int64_t first, int64_t count, int64_t every);
+
+/* Block until the stream is idle (queue.finish(), lbmcl.hpp:525-532). */
+int lbm_finish(lbm_ctx* ctx);
+
+/* Blocking reads into caller memory (lbmcl.hpp:159-166, 261-277).  map: dim^3
+ * int32 cell flags (common.h:7-20); rho: dim^3 values, u: 3*dim^3 values
+ * [ux | uy | uz], of the arithmetic type (float for shifted storage). */
+int lbm_read_map(lbm_ctx* ctx, int32_t* out);
+int lbm_read_rho(lbm_ctx* ctx, void* out);
+int lbm_read_u(lbm_ctx* ctx, void* out);
+
+/* Blocking read of lattice `buffer` (0: read by odd iterations, the
+ * reference's f_collide; 1: read by even iterations, its f_stream) in the
+ * reference's CSoA push view, 19*dim^3 values of the arithmetic type
+ * (lbmcl.hpp:206-214, 503, 517-519). */
+int lbm_read_f(lbm_ctx* ctx, int32_t buffer, void* out);
+
+/* Blocking read of lattice `buffer` as stored on the device (pull form, storage
+ * type: 4 or 8 bytes per slot native, 2 bytes shifted).  For checkpoints and
+ * tests; no counterpart in the reference. */
+int lbm_read_state(lbm_ctx* ctx, int32_t buffer, void* out);
+
+/* Blocking write of lattice `buffer` in stored (pull) form; restores a state
+ * read by lbm_read_state.  No counterpart in the reference. */
+int lbm_write_state(lbm_ctx* ctx, int32_t buffer, const void* in);
+
+/* Mass and kinetic energy of the last lbm_compute with diagnostics != 0
+ * (blocking). */
+int lbm_read_diagnostics(lbm_ctx* ctx, double* mass, double* kinetic_energy);
+
+/* Constants the kernels use, widened to double: out[0] = 1/tau, out[1] = U,
+ * out[2] = nu, out[3] = storage scale (1 for native storage). */
+int lbm_get_constants(lbm_ctx* ctx, double* out);
+
+/* Event timing (profile != 0), the reference's named cl::Event list
+ * (lbmcl.hpp:74, 545-593). */
+int lbm_event_count(lbm_ctx* ctx, int64_t* count);
+int lbm_event_get(lbm_ctx* ctx, int64_t index, int32_t* kind, double* ms);
+/* First event start to last event end (lbmcl.hpp:548-556). */
+int lbm_total_time_ms(lbm_ctx* ctx, double* ms);
+/* Sum of the compute events (lbmcl.hpp:562-574). */
+int lbm_kernels_time_ms(lbm_ctx* ctx, double* ms);
+/* Forget recorded events (keeps the context). */
+int lbm_clear_events(lbm_ctx* ctx);
+
+/* Text of the calling thread's last error ("" if none). */
+const char* lbm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,81 @@
+"""Build the native parts of lbmcl_amd in-tree.
+
+Outputs (git-ignored, they travel with the tree to the GPU host):
+  lbmcl_amd/liblbm_hip.so      HIP kernels + C ABI (include/lbm_hip.h), gfx950
+  lbmcl_amd/bin/lbmcl          the lbmcl command line (C++ host, links liblbm_hip.so)
+  lbmcl_amd/bin/lbm_io_selftest  writes the dump formats from synthetic arrays (CPU only)
+  lbmcl_amd/libmembw.so        plain HBM copy kernel used to calibrate bandwidth figures
+
+Usage: python -m lbmcl_amd.build [--only lib|cli|membw]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BIN = os.path.join(PKG, "bin")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Every HIP compile: gfx950 only, no packed-FP32 instructions, no contraction.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+]
+
+LIB = os.path.join(PKG, "liblbm_hip.so")
+MEMBW = os.path.join(PKG, "libmembw.so")
+CLI = os.path.join(BIN, "lbmcl")
+IO_SELFTEST = os.path.join(BIN, "lbm_io_selftest")
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib():
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
+          os.path.join(CSRC, "lbm_capi.hip"), "-o", LIB])
+
+
+def build_membw():
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "membw.hip"), "-o", MEMBW])
+
+
+def build_cli():
+    os.makedirs(BIN, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", INCLUDE, "-I", CSRC]
+    _run([cxx, *common, os.path.join(CSRC, "main.cpp"), "-o", CLI,
+          "-L", PKG, "-llbm_hip", "-Wl,-rpath,$ORIGIN/.."])
+    _run([cxx, *common, os.path.join(CSRC, "lbm_io_selftest.cpp"), "-o", IO_SELFTEST])
+
+
+def build(only=None):
+    if only in (None, "lib"):
+        build_lib()
+    if only in (None, "membw"):
+        build_membw()
+    if only in (None, "cli"):
+        build_cli()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--only", choices=["lib", "cli", "membw"])
+    args = ap.parse_args(argv)
+    build(args.only)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

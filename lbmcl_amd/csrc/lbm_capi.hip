@@ -1,0 +1,609 @@
+// lbm_capi.hip -- implementation of the C ABI declared in include/lbm_hip.h.
+//
+// Host side of liblbm_hip.so: device selection, buffers, the ping-pong
+// schedule of the compute kernel, blocking reads and event timing.  It takes
+// the place of the OpenCL device layer of LBMCL (lbmcl.hpp:392-593,
+// libs/CLUtil.hpp).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& what) {
+    g_last_error = what;
+    return code;
+}
+
+#define LBM_HIP_CHECK(call, what)                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(LBM_ERR_RUNTIME, std::string(what) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int ilog2(int64_t v) {
+    int k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+}
+
+// The reference passes nu and U to the OpenCL compiler as -D literals printed
+// by an ostream with 6 significant digits (lbmcl.hpp:140-141); in FP32 builds
+// the literal is single precision (lbmcl.hpp:146).  Reproduce that rounding.
+double kernel_literal(double value, int precision) {
+    char buf[64];
+    if (precision == LBM_F32) {
+        std::snprintf(buf, sizeof buf, "%g", double(float(value)));
+        return double(std::strtof(buf, nullptr));
+    }
+    std::snprintf(buf, sizeof buf, "%g", value);
+    return std::strtod(buf, nullptr);
+}
+
+// Power-of-two scale of the shifted storage: 2^floor(log2(1024 / (0.05 + 2|U|))),
+// clamped to [2^0, 2^24].  |f_i - w_i| stays far below 1 in the cavity, so
+// scaled values stay in binary16's normal range with ample headroom.
+double shift_scale(double U) {
+    const double a = 0.05 + 2.0 * std::fabs(U);
+    int k = int(std::floor(std::log2(1024.0 / a)));
+    if (k < 0) k = 0;
+    if (k > 24) k = 24;
+    return std::ldexp(1.0, k);
+}
+
+struct EventRec {
+    int32_t kind;
+    hipEvent_t start;
+    hipEvent_t stop;
+};
+
+}  // namespace
+
+struct lbm_ctx {
+    lbm_config cfg;
+    int32_t n = 0;
+    int32_t log2n = 0;
+    int64_t n3 = 0;
+    int64_t stride = 0;
+    size_t tsize = 0;   // arithmetic type size
+    size_t ssize = 0;   // storage element size
+    hipStream_t stream = nullptr;
+    void* lat[2] = {nullptr, nullptr};  // lat[0] read by odd iterations, lat[1] by even
+    void* rho = nullptr;
+    void* u = nullptr;
+    int32_t* map = nullptr;
+    double* partials = nullptr;
+    int64_t partials_cap = 0;  // in blocks
+    double* diag = nullptr;    // 2 doubles
+    bool diag_valid = false;
+    bool initialized = false;
+    double nu = 0, inv_tau = 0, U = 0, scale = 1;
+    std::vector<EventRec> events;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+int set_device(lbm_ctx* c) {
+    LBM_HIP_CHECK(hipSetDevice(c->cfg.device), "hipSetDevice");
+    return LBM_OK;
+}
+
+int ev_begin(lbm_ctx* c, int kind, EventRec& r) {
+    r.kind = kind;
+    r.start = r.stop = nullptr;
+    if (!c->cfg.profile) return LBM_OK;
+    LBM_HIP_CHECK(hipEventCreate(&r.start), "hipEventCreate");
+    LBM_HIP_CHECK(hipEventCreate(&r.stop), "hipEventCreate");
+    LBM_HIP_CHECK(hipEventRecord(r.start, c->stream), "hipEventRecord");
+    return LBM_OK;
+}
+
+int ev_end(lbm_ctx* c, EventRec& r) {
+    if (!c->cfg.profile) return LBM_OK;
+    LBM_HIP_CHECK(hipEventRecord(r.stop, c->stream), "hipEventRecord");
+    c->events.push_back(r);
+    return LBM_OK;
+}
+
+void free_events(lbm_ctx* c) {
+    for (auto& e : c->events) {
+        if (e.start) (void)hipEventDestroy(e.start);
+        if (e.stop) (void)hipEventDestroy(e.stop);
+    }
+    c->events.clear();
+}
+
+void release(lbm_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_events(c);
+    for (void* p : {c->lat[0], c->lat[1], c->rho, c->u, (void*)c->map, (void*)c->partials, (void*)c->diag})
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int dev_alloc(void** p, size_t bytes, const char* what) {
+    *p = nullptr;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(LBM_ERR_MEMORY, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
+                                        " bytes): " + hipGetErrorString(e));
+    }
+    return LBM_OK;
+}
+
+bool use_row_kernel(const lbm_ctx* c) {
+    if (c->cfg.kernel == LBM_KERNEL_GENERIC) return false;
+    return c->n >= 64;
+}
+
+int generic_block(const lbm_ctx* c) {
+    int64_t b = 1;
+    for (int d = 0; d < 3; ++d) b *= (c->cfg.lws[d] > 0 ? c->cfg.lws[d] : 1);
+    if (c->cfg.lws[0] <= 0 && c->cfg.lws[1] <= 0 && c->cfg.lws[2] <= 0) b = 256;
+    return int(b);
+}
+
+// Number of blocks of one compute launch (for the diagnostics partials).
+int64_t compute_blocks(const lbm_ctx* c) {
+    if (use_row_kernel(c)) {
+        const int64_t bx = c->n < 256 ? c->n : 256;
+        return (c->n / bx) * int64_t(c->n - 2) * int64_t(c->n - 2);
+    }
+    const int64_t b = generic_block(c);
+    return (c->n3 + b - 1) / b;
+}
+
+template <class T, int KIND>
+int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
+    lbm::StepArgs<T> a{};
+    a.src = src;
+    a.dst = dst;
+    a.rho = static_cast<T*>(c->rho);
+    a.u = static_cast<T*>(c->u);
+    a.partials = c->partials;
+    a.n3 = c->n3;
+    a.stride = c->stride;
+    a.n = c->n;
+    a.log2n = c->log2n;
+    a.update_macro = update_macro;
+    a.diagnostics = diag;
+    a.xcd_remap = c->cfg.xcd_remap;
+    a.k.inv_tau = T(c->inv_tau);
+    a.k.U = T(c->U);
+    a.scale = float(c->scale);
+    a.inv_scale = float(1.0 / c->scale);
+    if (use_row_kernel(c)) {
+        const int bx = c->n < 256 ? c->n : 256;
+        a.gx = c->n / bx;
+        const int64_t nb = int64_t(a.gx) * int64_t(c->n - 2) * int64_t(c->n - 2);
+        hipLaunchKernelGGL((lbm::lbm_collide_stream_row<T, KIND>), dim3(unsigned(nb)), dim3(bx), 0, c->stream, a);
+    } else {
+        const int b = generic_block(c);
+        const int64_t nb = (c->n3 + b - 1) / b;
+        a.gx = 0;
+        hipLaunchKernelGGL((lbm::lbm_collide_stream_generic<T, KIND>), dim3(unsigned(nb)), dim3(b), 0, c->stream,
+                           a);
+    }
+    LBM_HIP_CHECK(hipGetLastError(), "compute launch");
+    return LBM_OK;
+}
+
+template <class T, int KIND>
+int launch_initialize(lbm_ctx* c) {
+    lbm::InitArgs<T> a{};
+    a.a = c->lat[0];
+    a.b = c->lat[1];
+    a.rho = static_cast<T*>(c->rho);
+    a.u = static_cast<T*>(c->u);
+    a.map = c->map;
+    a.n3 = c->n3;
+    a.stride = c->stride;
+    a.n = c->n;
+    a.log2n = c->log2n;
+    a.U = T(c->U);
+    a.scale = float(c->scale);
+    a.inv_scale = float(1.0 / c->scale);
+    const int64_t nb = (c->n3 + 255) / 256;
+    hipLaunchKernelGGL((lbm::lbm_initialize_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
+    LBM_HIP_CHECK(hipGetLastError(), "initialize launch");
+    return LBM_OK;
+}
+
+template <class T, int KIND>
+int launch_view(lbm_ctx* c, const void* p, T* out) {
+    lbm::ViewArgs<T> a{};
+    a.p = p;
+    a.out = out;
+    a.n3 = c->n3;
+    a.stride = c->stride;
+    a.n = c->n;
+    a.log2n = c->log2n;
+    a.scale = float(c->scale);
+    a.inv_scale = float(1.0 / c->scale);
+    const int64_t nb = (c->n3 + 255) / 256;
+    hipLaunchKernelGGL((lbm::lbm_view_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
+    LBM_HIP_CHECK(hipGetLastError(), "view launch");
+    return LBM_OK;
+}
+
+// Dispatch on (precision, storage).
+#define LBM_DISPATCH(c, FN, ...)                                                        \
+    ((c)->cfg.precision == LBM_F64 ? FN<double, lbm::ST_NATIVE>(__VA_ARGS__)            \
+     : (c)->cfg.storage == LBM_STORE_FP16_SHIFT ? FN<float, lbm::ST_FP16>(__VA_ARGS__)  \
+     : (c)->cfg.storage == LBM_STORE_BF16_SHIFT ? FN<float, lbm::ST_BF16>(__VA_ARGS__)  \
+                                                : FN<float, lbm::ST_NATIVE>(__VA_ARGS__))
+
+int read_device(lbm_ctx* c, int kind, void* dst, const void* src, size_t bytes) {
+    EventRec r;
+    int rc = ev_begin(c, kind, r);
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync D2H");
+    rc = ev_end(c, r);
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+#define LBM_CHECK_CTX(c)                                                   \
+    do {                                                                   \
+        if (!(c)) return fail(LBM_ERR_INVALID, "null context");            \
+        int rc_ = set_device(c);                                           \
+        if (rc_) return rc_;                                               \
+    } while (0)
+
+#define LBM_CHECK_INIT(c)                                                                \
+    do {                                                                                 \
+        if (!(c)->initialized) return fail(LBM_ERR_STATE, "lbm_initialize not called");  \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+int lbm_get_device_count(int32_t* count) {
+    if (!count) return fail(LBM_ERR_INVALID, "null count");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        if (e == hipErrorNoDevice) return LBM_OK;
+        return fail(LBM_ERR_RUNTIME, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    *count = n;
+    return LBM_OK;
+}
+
+int lbm_get_device_name(int32_t device, char* name, size_t len) {
+    if (!name || len == 0) return fail(LBM_ERR_INVALID, "null name buffer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return fail(LBM_ERR_DEVICE, "no HIP device " + std::to_string(device));
+    }
+    hipDeviceProp_t prop;
+    LBM_HIP_CHECK(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    std::snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+    return LBM_OK;
+}
+
+int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
+    if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const lbm_config& k = *cfg;
+    if (!is_pow2(k.dim)) return fail(LBM_ERR_INVALID, "dim must be a positive power of two");
+    if (k.dim > 1024) return fail(LBM_ERR_INVALID, "dim above 1024 is not supported");
+    const int64_t n3 = int64_t(k.dim) * k.dim * k.dim;
+    if (!is_pow2(k.stride) || k.stride > n3)
+        return fail(LBM_ERR_INVALID, "stride must be a power of two in [1, dim^3]");
+    if (k.precision != LBM_F32 && k.precision != LBM_F64) return fail(LBM_ERR_INVALID, "unknown precision");
+    if (k.storage != LBM_STORE_NATIVE && k.storage != LBM_STORE_FP16_SHIFT && k.storage != LBM_STORE_BF16_SHIFT)
+        return fail(LBM_ERR_INVALID, "unknown storage");
+    if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
+        return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
+    if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW)
+        return fail(LBM_ERR_INVALID, "unknown kernel selection");
+    if (k.kernel == LBM_KERNEL_ROW && k.dim < 64) return fail(LBM_ERR_INVALID, "the row kernel needs dim >= 64");
+    if (!(k.viscosity >= 0.0) || !std::isfinite(k.viscosity)) return fail(LBM_ERR_INVALID, "invalid viscosity");
+    if (!std::isfinite(k.velocity)) return fail(LBM_ERR_INVALID, "invalid velocity");
+    int64_t lws = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (k.lws[d] < 0) return fail(LBM_ERR_INVALID, "negative work-group size");
+        lws *= (k.lws[d] > 0 ? k.lws[d] : 1);
+    }
+    if (lws > 1024) return fail(LBM_ERR_INVALID, "work-group size above 1024");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) {
+        (void)hipGetLastError();
+        ndev = 0;
+    }
+    if (k.device < 0 || k.device >= ndev)
+        return fail(LBM_ERR_DEVICE, "no HIP device " + std::to_string(k.device) + " (" + std::to_string(ndev) +
+                                        " present)");
+
+    lbm_ctx* c = new (std::nothrow) lbm_ctx();
+    if (!c) return fail(LBM_ERR_MEMORY, "host allocation failed");
+    c->cfg = k;
+    c->n = k.dim;
+    c->log2n = ilog2(k.dim);
+    c->n3 = n3;
+    c->stride = k.stride;
+    c->tsize = (k.precision == LBM_F64) ? 8 : 4;
+    c->ssize = (k.storage == LBM_STORE_NATIVE) ? c->tsize : 2;
+    c->nu = kernel_literal(k.viscosity, k.precision);
+    c->U = kernel_literal(k.velocity, k.precision);
+    if (k.precision == LBM_F32) {
+        const float nu = float(c->nu);
+        const float tau = (3.0f * nu) + 0.5f;  // kernels.cl:61, folded in float
+        c->inv_tau = double(1.0f / tau);       // kernels.cl:62
+    } else {
+        const double tau = (3.0 * c->nu) + 0.5;
+        c->inv_tau = 1.0 / tau;
+    }
+    c->scale = (k.storage == LBM_STORE_NATIVE) ? 1.0 : shift_scale(c->U);
+
+    int rc = set_device(c);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        c->stream = nullptr;
+        release(c);
+        return fail(LBM_ERR_RUNTIME, "hipStreamCreate failed");
+    }
+    const size_t lat_bytes = size_t(n3) * lbm::Q * c->ssize;
+    c->partials_cap = compute_blocks(c);
+    if ((rc = dev_alloc(&c->lat[0], lat_bytes, "lattice 0")) || (rc = dev_alloc(&c->lat[1], lat_bytes, "lattice 1")) ||
+        (rc = dev_alloc(&c->rho, size_t(n3) * c->tsize, "rho")) ||
+        (rc = dev_alloc(&c->u, size_t(3 * n3) * c->tsize, "u")) ||
+        (rc = dev_alloc(reinterpret_cast<void**>(&c->map), size_t(n3) * 4, "map")) ||
+        (rc = dev_alloc(reinterpret_cast<void**>(&c->partials), size_t(c->partials_cap) * 2 * sizeof(double),
+                        "partials")) ||
+        (rc = dev_alloc(reinterpret_cast<void**>(&c->diag), 2 * sizeof(double), "diagnostics"))) {
+        std::string msg = g_last_error;
+        release(c);
+        return fail(rc, msg);
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* ctx) {
+    if (!ctx) return fail(LBM_ERR_INVALID, "null context");
+    release(ctx);
+    return LBM_OK;
+}
+
+int lbm_initialize(lbm_ctx* c) {
+    LBM_CHECK_CTX(c);
+    EventRec r;
+    int rc = ev_begin(c, LBM_EVENT_INITIALIZE, r);
+    if (rc) return rc;
+    rc = LBM_DISPATCH(c, launch_initialize, c);
+    if (rc) return rc;
+    rc = ev_end(c, r);
+    if (rc) return rc;
+    c->initialized = true;
+    c->diag_valid = false;
+    return LBM_OK;
+}
+
+int lbm_compute(lbm_ctx* c, int64_t iteration, int32_t update_macro, int32_t diagnostics) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (iteration < 1) return fail(LBM_ERR_INVALID, "iterations are numbered from 1");
+    const bool odd = (iteration % 2) != 0;  // lbmcl.hpp:437, 456-457
+    const void* src = odd ? c->lat[0] : c->lat[1];
+    void* dst = odd ? c->lat[1] : c->lat[0];
+    EventRec r;
+    int rc = ev_begin(c, LBM_EVENT_COMPUTE, r);
+    if (rc) return rc;
+    rc = LBM_DISPATCH(c, launch_compute, c, src, dst, update_macro ? 1 : 0, diagnostics ? 1 : 0);
+    if (rc) return rc;
+    rc = ev_end(c, r);
+    if (rc) return rc;
+    if (diagnostics) {
+        EventRec d;
+        rc = ev_begin(c, LBM_EVENT_DIAGNOSTICS, d);
+        if (rc) return rc;
+        hipLaunchKernelGGL(lbm::lbm_reduce_partials, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                           compute_blocks(c), c->diag);
+        LBM_HIP_CHECK(hipGetLastError(), "diagnostics reduction launch");
+        rc = ev_end(c, d);
+        if (rc) return rc;
+        c->diag_valid = true;
+    }
+    return LBM_OK;
+}
+
+int lbm_compute_range(lbm_ctx* c, int64_t first, int64_t count, int64_t every) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (first < 1 || count < 0 || every < 0) return fail(LBM_ERR_INVALID, "bad iteration range");
+    for (int64_t it = first; it < first + count; ++it) {
+        const int um = (every > 0 && it % every == 0) ? 1 : 0;
+        const int rc = lbm_compute(c, it, um, 0);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_finish(lbm_ctx* c) {
+    LBM_CHECK_CTX(c);
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int lbm_read_map(lbm_ctx* c, int32_t* out) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    return read_device(c, LBM_EVENT_READ_MAP, out, c->map, size_t(c->n3) * 4);
+}
+
+int lbm_read_rho(lbm_ctx* c, void* out) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    return read_device(c, LBM_EVENT_READ_RHO, out, c->rho, size_t(c->n3) * c->tsize);
+}
+
+int lbm_read_u(lbm_ctx* c, void* out) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    return read_device(c, LBM_EVENT_READ_U, out, c->u, size_t(3 * c->n3) * c->tsize);
+}
+
+int lbm_read_f(lbm_ctx* c, int32_t buffer, void* out) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    if (buffer != 0 && buffer != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
+    const size_t bytes = size_t(c->n3) * lbm::Q * c->tsize;
+    void* tmp = nullptr;
+    int rc = dev_alloc(&tmp, bytes, "f view");
+    if (rc) return rc;
+    EventRec r;
+    rc = ev_begin(c, LBM_EVENT_READ_F, r);
+    if (!rc) {
+        if (c->cfg.precision == LBM_F64)
+            rc = launch_view<double, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<double*>(tmp));
+        else if (c->cfg.storage == LBM_STORE_FP16_SHIFT)
+            rc = launch_view<float, lbm::ST_FP16>(c, c->lat[buffer], static_cast<float*>(tmp));
+        else if (c->cfg.storage == LBM_STORE_BF16_SHIFT)
+            rc = launch_view<float, lbm::ST_BF16>(c, c->lat[buffer], static_cast<float*>(tmp));
+        else
+            rc = launch_view<float, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<float*>(tmp));
+    }
+    if (!rc) {
+        const hipError_t e = hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) rc = fail(LBM_ERR_RUNTIME, std::string("hipMemcpyAsync D2H: ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = ev_end(c, r);
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (rc) return rc;
+    if (se != hipSuccess) return fail(LBM_ERR_RUNTIME, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+    return LBM_OK;
+}
+
+int lbm_read_state(lbm_ctx* c, int32_t buffer, void* out) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!out) return fail(LBM_ERR_INVALID, "null output");
+    if (buffer != 0 && buffer != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
+    return read_device(c, LBM_EVENT_READ_F, out, c->lat[buffer], size_t(c->n3) * lbm::Q * c->ssize);
+}
+
+int lbm_write_state(lbm_ctx* c, int32_t buffer, const void* in) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!in) return fail(LBM_ERR_INVALID, "null input");
+    if (buffer != 0 && buffer != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
+    LBM_HIP_CHECK(hipMemcpyAsync(c->lat[buffer], in, size_t(c->n3) * lbm::Q * c->ssize, hipMemcpyHostToDevice,
+                                 c->stream),
+                  "hipMemcpyAsync H2D");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int lbm_read_diagnostics(lbm_ctx* c, double* mass, double* kinetic_energy) {
+    LBM_CHECK_CTX(c);
+    if (!mass || !kinetic_energy) return fail(LBM_ERR_INVALID, "null output");
+    if (!c->diag_valid) return fail(LBM_ERR_STATE, "no compute with diagnostics enqueued");
+    double h[2];
+    LBM_HIP_CHECK(hipMemcpyAsync(h, c->diag, sizeof h, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync D2H");
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    *mass = h[0];
+    *kinetic_energy = h[1];
+    return LBM_OK;
+}
+
+int lbm_get_constants(lbm_ctx* c, double* out) {
+    if (!c || !out) return fail(LBM_ERR_INVALID, "null argument");
+    out[0] = c->inv_tau;
+    out[1] = c->U;
+    out[2] = c->nu;
+    out[3] = c->scale;
+    return LBM_OK;
+}
+
+int lbm_event_count(lbm_ctx* c, int64_t* count) {
+    if (!c || !count) return fail(LBM_ERR_INVALID, "null argument");
+    *count = int64_t(c->events.size());
+    return LBM_OK;
+}
+
+int lbm_event_get(lbm_ctx* c, int64_t index, int32_t* kind, double* ms) {
+    LBM_CHECK_CTX(c);
+    if (index < 0 || index >= int64_t(c->events.size())) return fail(LBM_ERR_INVALID, "event index out of range");
+    const EventRec& e = c->events[size_t(index)];
+    LBM_HIP_CHECK(hipEventSynchronize(e.stop), "hipEventSynchronize");
+    float t = 0.f;
+    LBM_HIP_CHECK(hipEventElapsedTime(&t, e.start, e.stop), "hipEventElapsedTime");
+    if (kind) *kind = e.kind;
+    if (ms) *ms = double(t);
+    return LBM_OK;
+}
+
+int lbm_total_time_ms(lbm_ctx* c, double* ms) {
+    LBM_CHECK_CTX(c);
+    if (!ms) return fail(LBM_ERR_INVALID, "null output");
+    *ms = 0.0;
+    if (c->events.empty()) return LBM_OK;
+    LBM_HIP_CHECK(hipEventSynchronize(c->events.back().stop), "hipEventSynchronize");
+    float t = 0.f;
+    LBM_HIP_CHECK(hipEventElapsedTime(&t, c->events.front().start, c->events.back().stop), "hipEventElapsedTime");
+    *ms = double(t);
+    return LBM_OK;
+}
+
+int lbm_kernels_time_ms(lbm_ctx* c, double* ms) {
+    LBM_CHECK_CTX(c);
+    if (!ms) return fail(LBM_ERR_INVALID, "null output");
+    double total = 0.0;
+    for (const EventRec& e : c->events) {
+        if (e.kind != LBM_EVENT_COMPUTE) continue;
+        LBM_HIP_CHECK(hipEventSynchronize(e.stop), "hipEventSynchronize");
+        float t = 0.f;
+        LBM_HIP_CHECK(hipEventElapsedTime(&t, e.start, e.stop), "hipEventElapsedTime");
+        total += double(t);
+    }
+    *ms = total;
+    return LBM_OK;
+}
+
+int lbm_clear_events(lbm_ctx* c) {
+    LBM_CHECK_CTX(c);
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    free_events(c);
+    return LBM_OK;
+}
+
+}  // extern "C"

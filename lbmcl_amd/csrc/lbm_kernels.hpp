@@ -1,0 +1,518 @@
+// lbm_kernels.hpp -- gfx950 kernels of the D3Q19 lid-driven cavity.
+//
+// Pull ("gather") form of the reference's push kernel (kernels.cl:321-426):
+// lattice P holds post-collision populations at their source cell, so a cell d
+// reads g_i = P[d - e_i][i] (misaligned by at most one element along x) and
+// writes its own 19 slots aligned.  Both lattices start as
+// P0[s][i] = F0[s + e_i][i] (F0 = the reference's initial lattice) and wall
+// cells are never written, so a slot whose source is a wall keeps supplying
+// the reference's stale initial value (kernels.cl:293-317, 421).  The gathered
+// populations, and hence rho, u and every later state, equal the reference's.
+//
+// One thread per cell.  HBM-bound (152 B/cell/step in FP32, 304 in FP64, 76
+// with 16-bit shifted storage); no MFMA -- nothing here is a contraction.
+// Everything is compiled without floating-point contraction, so each value is
+// the reference's expression evaluated operation by operation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "lbm_lattice.hpp"
+
+#pragma clang fp contract(off)
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Storage codecs.  Native stores T.  The shifted forms store
+// h = (f - w_i) * scale rounded to nearest even in binary16 or bfloat16 and
+// decode f = w_i + h * (1/scale); scale is a power of two, so both products are
+// exact and only the subtraction/addition and the 16-bit rounding round.
+// ---------------------------------------------------------------------------
+enum StorageKind { ST_NATIVE = 0, ST_FP16 = 1, ST_BF16 = 2 };
+
+template <class T, int KIND> struct Codec;
+
+template <class T> struct Codec<T, ST_NATIVE> {
+    using S = T;
+    float scale, inv_scale;
+    __device__ __forceinline__ T load(const S* p, T) const { return *p; }
+    __device__ __forceinline__ void store(S* p, T, T v) const { __builtin_nontemporal_store(v, p); }
+};
+
+template <> struct Codec<float, ST_FP16> {
+    using S = _Float16;
+    float scale, inv_scale;
+    __device__ __forceinline__ float load(const S* p, float w) const { return w + float(*p) * inv_scale; }
+    __device__ __forceinline__ void store(S* p, float w, float v) const {
+        const _Float16 h = _Float16((v - w) * scale);
+        __builtin_nontemporal_store(h, p);
+    }
+};
+
+template <> struct Codec<float, ST_BF16> {
+    using S = uint16_t;
+    float scale, inv_scale;
+    __device__ __forceinline__ float load(const S* p, float w) const {
+        return w + __uint_as_float(uint32_t(*p) << 16) * inv_scale;
+    }
+    __device__ __forceinline__ void store(S* p, float w, float v) const {
+        // round to nearest even; NaN stays a quiet NaN
+        const uint32_t x = __float_as_uint((v - w) * scale);
+        const uint16_t h = ((x & 0x7fffffffu) > 0x7f800000u) ? uint16_t((x >> 16) | 0x40u)
+                                                               : uint16_t((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+        __builtin_nontemporal_store(h, p);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Lattice constants in the arithmetic type (kernels.cl:110-128; with
+// -cl-single-precision-constant the reference folds them in float for FP32).
+// ---------------------------------------------------------------------------
+template <class T> struct W {
+    static constexpr T w0 = T(1) / T(3);
+    static constexpr T w1 = T(1) / T(18);
+    static constexpr T w7 = T(1) / T(36);
+};
+
+__host__ __device__ constexpr int ex_of(int i) {
+    return (i == 1 || i == 7 || i == 10 || i == 11 || i == 15) ? 1
+           : (i == 3 || i == 8 || i == 9 || i == 13 || i == 17) ? -1
+                                                                 : 0;
+}
+__host__ __device__ constexpr int ey_of(int i) {
+    return (i == 2 || i == 7 || i == 8 || i == 12 || i == 16) ? 1
+           : (i == 4 || i == 9 || i == 10 || i == 14 || i == 18) ? -1
+                                                                  : 0;
+}
+__host__ __device__ constexpr int ez_of(int i) {
+    return (i == 6 || i == 15 || i == 16 || i == 17 || i == 18) ? 1
+           : (i == 5 || i == 11 || i == 12 || i == 13 || i == 14) ? -1
+                                                                   : 0;
+}
+template <class T> __device__ __forceinline__ T w_of(int i) {
+    return i == 0 ? W<T>::w0 : (i <= 6 ? W<T>::w1 : W<T>::w7);
+}
+
+template <class T> struct StepConsts {
+    T inv_tau;  // 1 / (3 nu + 1/2) folded in T (kernels.cl:61-62)
+    T U;        // lid velocity literal (kernels.cl:57)
+};
+
+// e_i . u with the reference's ((ux ex + uy ey) + uz ez) evaluated exactly:
+// products with 0 contribute +-0 and products with -1 are negations, so the
+// expression reduces to the signed sum of the non-zero components in order.
+template <class T> __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
+    switch (i) {
+        case 0: return T(0);
+        case 1: return ux;
+        case 2: return uy;
+        case 3: return -ux;
+        case 4: return -uy;
+        case 5: return -uz;
+        case 6: return uz;
+        case 7: return ux + uy;
+        case 8: return uy - ux;
+        case 9: return -(ux + uy);
+        case 10: return ux - uy;
+        case 11: return ux - uz;
+        case 12: return uy - uz;
+        case 13: return -(ux + uz);
+        case 14: return -(uy + uz);
+        case 15: return ux + uz;
+        case 16: return uy + uz;
+        case 17: return uz - ux;
+        default: return uz - uy;
+    }
+}
+
+// (rho w_i) (1 + 3 eu + 4.5 eu eu - 1.5 u2), left to right (kernels.cl:397, 416).
+template <class T> __device__ __forceinline__ T feq(int i, T rho, T ux, T uy, T uz, T u2) {
+    const T eu = edot<T>(i, ux, uy, uz);
+    return (rho * w_of<T>(i)) * (((T(1) + (T(3) * eu)) + ((T(4.5) * eu) * eu)) - (T(1.5) * u2));
+}
+
+// Local part of `compute` (kernels.cl:343-418) for a non-wall cell of type t.
+// On entry f holds the gathered populations, on return the ones to store.
+template <class T>
+__device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+                                        const StepConsts<T>& k) {
+    const bool moving = is_moving(t);
+    if (moving) {  // kernels.cl:343-349
+        f[5] = f[6];
+        f[11] = f[17];
+        f[12] = f[18];
+        f[13] = f[15];
+        f[14] = f[16];
+    }
+    rho = f[0];  // kernels.cl:355
+#pragma unroll
+    for (int i = 1; i < Q; ++i) rho = rho + f[i];
+    if (moving) {  // kernels.cl:362-365
+        ux = k.U;
+        uy = T(0);
+        uz = T(0);
+    } else {  // kernels.cl:376-378
+        const T jx = ((((f[1] + f[7]) + f[10]) + f[11]) + f[15]) - ((((f[3] + f[8]) + f[9]) + f[13]) + f[17]);
+        const T jy = ((((f[2] + f[7]) + f[8]) + f[12]) + f[16]) - ((((f[4] + f[9]) + f[10]) + f[14]) + f[18]);
+        const T jz = ((((f[6] + f[15]) + f[16]) + f[17]) + f[18]) - ((((f[5] + f[11]) + f[12]) + f[13]) + f[14]);
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = jz / rho;
+    }
+    const T u2 = ((ux * ux) + (uy * uy)) + (uz * uz);  // kernels.cl:390
+    if (moving) {
+        // kernels.cl:393-398 sets f_i = feq_i; the BGK step that follows
+        // (kernels.cl:412-418) then adds inv_tau * (feq_i - f_i) = 0 exactly.
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = feq<T>(i, rho, ux, uy, uz, u2);
+    } else if (t == FLUID) {  // kernels.cl:412-418, 270-273
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T e = feq<T>(i, rho, ux, uy, uz, u2);
+            f[i] = f[i] + k.inv_tau * (e - f[i]);
+        }
+    } else if (is_bounceback(t)) {  // kernels.cl:400-408
+        T tmp;
+        tmp = f[1]; f[1] = f[3]; f[3] = tmp;
+        tmp = f[2]; f[2] = f[4]; f[4] = tmp;
+        tmp = f[5]; f[5] = f[6]; f[6] = tmp;
+        tmp = f[7]; f[7] = f[9]; f[9] = tmp;
+        tmp = f[8]; f[8] = f[10]; f[10] = tmp;
+        tmp = f[11]; f[11] = f[17]; f[17] = tmp;
+        tmp = f[12]; f[12] = f[18]; f[18] = tmp;
+        tmp = f[13]; f[13] = f[15]; f[15] = tmp;
+        tmp = f[14]; f[14] = f[16]; f[16] = tmp;
+    }
+    // CORNER: pure propagation (kernels.cl:254-260 with no BC and no collision).
+}
+
+// ---------------------------------------------------------------------------
+// Block reduction of (mass, kinetic energy) in FP64: wavefront shuffles, then
+// LDS across the block's wavefronts.  Result valid in thread 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_reduce2(double& a, double& b, double* lds) {
+    if (blockDim.x & 63u) {
+        // block sizes that are not whole wavefronts: serial sum through LDS
+        __syncthreads();
+        lds[2 * threadIdx.x] = a;
+        lds[2 * threadIdx.x + 1] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sa = lds[0], sb = lds[1];
+            for (uint32_t j = 1; j < blockDim.x; ++j) {
+                sa += lds[2 * j];
+                sb += lds[2 * j + 1];
+            }
+            a = sa;
+            b = sb;
+        }
+        return;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        lds[2 * wave] = a;
+        lds[2 * wave + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = lds[0], sb = lds[1];
+        for (int w = 1; w < nwaves; ++w) {
+            sa += lds[2 * w];
+            sb += lds[2 * w + 1];
+        }
+        a = sa;
+        b = sb;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void diag_terms(T rho, T ux, T uy, T uz, double& m, double& e) {
+    const double dr = double(rho), dx = double(ux), dy = double(uy), dz = double(uz);
+    m = dr;
+    e = 0.5 * dr * (((dx * dx) + (dy * dy)) + (dz * dz));
+}
+
+// ---------------------------------------------------------------------------
+// Kernel arguments.
+// ---------------------------------------------------------------------------
+template <class T> struct StepArgs {
+    const void* src;      // lattice read by this iteration (pull form)
+    void* dst;            // lattice written
+    T* rho;               // N^3
+    T* u;                 // 3 N^3, [ux | uy | uz]
+    double* partials;     // 2 doubles per block (diagnostics)
+    int64_t n3;           // N^3
+    int64_t stride;       // CSoA stride S
+    int32_t n;            // N
+    int32_t log2n;
+    int32_t update_macro;
+    int32_t diagnostics;
+    int32_t xcd_remap;
+    int32_t gx;           // row kernel: blocks per row
+    StepConsts<T> k;
+    float scale, inv_scale;
+};
+
+// ---------------------------------------------------------------------------
+// Row kernel (N >= 64).  A block covers blockDim.x consecutive cells of one
+// (y, z) row with 1 <= y, z <= N-2; wall rows are not launched.  Because
+// slot(c, q) = v(x) + slot(N r, q) with v(x) = 19 x - 18 (x mod S) for
+// c = x + N r, every gather address is a per-row (wavefront-uniform, scalar)
+// 64-bit base plus one of three small per-thread offsets v(x-1), v(x), v(x+1):
+// the loads use a scalar base and a 32-bit vector offset.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb, int remap) {
+    if (!remap) return b;
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of logical blocks so neighbouring rows share an L2.
+    const uint32_t q = nb >> 3, r = nb & 7u;
+    const uint32_t xcd = b & 7u, k = b >> 3;
+    return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+template <class T, int KIND>
+__global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    const uint32_t gx = uint32_t(a.gx);
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    const uint32_t xb = lb % gx;
+    const uint32_t row = lb / gx;                   // 0 .. (N-2)^2 - 1
+    const int32_t y = int32_t(row % uint32_t(n - 2)) + 1;
+    const int32_t z = int32_t(row / uint32_t(n - 2)) + 1;
+    const int32_t x = int32_t(xb * blockDim.x + threadIdx.x);
+    const int64_t r = int64_t(y) + int64_t(n) * z;  // row index of cell c = x + N r
+    const int32_t t = finish_type(x_flags(x, n) | row_flags(y, z, n));
+    const bool active = (t != WALL);
+    const int64_t smask = a.stride - 1;
+    const uint32_t smask32 = uint32_t(smask);
+
+    // per-thread offsets v(x - ex) for ex = +1, 0, -1 (valid for active lanes)
+    const uint32_t xm = uint32_t(x - 1), x0 = uint32_t(x), xp = uint32_t(x + 1);
+    const uint32_t v_m = 19u * xm - 18u * (xm & smask32);
+    const uint32_t v_0 = 19u * x0 - 18u * (x0 & smask32);
+    const uint32_t v_p = 19u * xp - 18u * (xp & smask32);
+
+    const S* __restrict__ src = static_cast<const S*>(a.src);
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+
+    T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
+    if (active) {
+        T f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i), ez = ez_of(i);
+            const int64_t c0 = int64_t(n) * (r - ey - int64_t(n) * ez);  // first cell of the source row
+            const int64_t base = 19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride;
+            const uint32_t off = ex == 1 ? v_m : (ex == -1 ? v_p : v_0);
+            f[i] = codec.load(src + base + off, w_of<T>(i));
+        }
+        collide<T>(t, f, rho, ux, uy, uz, a.k);
+        const int64_t c0 = int64_t(n) * r;
+        const int64_t obase = 19 * c0 - 18 * (c0 & smask);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) codec.store(dst + obase + int64_t(i) * a.stride + v_0, w_of<T>(i), f[i]);
+        if (a.update_macro && is_store_macro(t)) {
+            const int64_t c = int64_t(x) + int64_t(n) * r;
+            a.rho[c] = rho;
+            a.u[c] = ux;
+            a.u[a.n3 + c] = uy;
+            a.u[2 * a.n3 + c] = uz;
+        }
+    }
+    if (a.diagnostics) {
+        __shared__ double lds[2 * 16];
+        double m = 0.0, e = 0.0;
+        if (active && is_store_macro(t)) diag_terms<T>(rho, ux, uy, uz, m, e);
+        block_reduce2(m, e, lds);
+        if (threadIdx.x == 0) {
+            a.partials[2 * lb] = m;
+            a.partials[2 * lb + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic kernel: any N, any stride, any block size.  One thread per cell of a
+// 1-D grid over all N^3 cells, full 64-bit slot arithmetic per population.
+// ---------------------------------------------------------------------------
+template <class T, int KIND>
+__global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool inside = c < a.n3;
+    const int32_t x = int32_t(c & (n - 1));
+    const int32_t y = int32_t((c >> a.log2n) & (n - 1));
+    const int32_t z = int32_t(c >> (2 * a.log2n));
+    const int32_t t = inside ? cell_type(x, y, z, n) : WALL;
+    const bool active = (t != WALL);
+    const S* __restrict__ src = static_cast<const S*>(a.src);
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+    T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
+    if (active) {
+        T f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
+            f[i] = codec.load(src + slot(s, i, a.stride), w_of<T>(i));
+        }
+        collide<T>(t, f, rho, ux, uy, uz, a.k);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) codec.store(dst + slot(c, i, a.stride), w_of<T>(i), f[i]);
+        if (a.update_macro && is_store_macro(t)) {
+            a.rho[c] = rho;
+            a.u[c] = ux;
+            a.u[a.n3 + c] = uy;
+            a.u[2 * a.n3 + c] = uz;
+        }
+    }
+    if (a.diagnostics) {
+        __shared__ double lds[2 * 1024];
+        double m = 0.0, e = 0.0;
+        if (active && is_store_macro(t)) diag_terms<T>(rho, ux, uy, uz, m, e);
+        block_reduce2(m, e, lds);
+        if (threadIdx.x == 0) {
+            a.partials[2 * blockIdx.x] = m;
+            a.partials[2 * blockIdx.x + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// initialize (kernels.cl:277-318) in pull form: cell s writes map, rho, u and
+// P0[s][i] = F0[s + e_i][i] into both lattices (NaN where s + e_i is outside
+// the lattice or a wall).
+// ---------------------------------------------------------------------------
+template <class T> struct InitArgs {
+    void* a;
+    void* b;
+    T* rho;
+    T* u;
+    int32_t* map;
+    int64_t n3;
+    int64_t stride;
+    int32_t n;
+    int32_t log2n;
+    T U;
+    float scale, inv_scale;
+};
+
+template <class T, int KIND>
+__global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= a.n3) return;
+    const int32_t x = int32_t(c & (n - 1));
+    const int32_t y = int32_t((c >> a.log2n) & (n - 1));
+    const int32_t z = int32_t(c >> (2 * a.log2n));
+    const int32_t t = cell_type(x, y, z, n);
+    a.map[c] = t;
+    const T qnan = __builtin_nan("");
+    const bool st = is_store_macro(t);
+    a.rho[c] = st ? T(1) : qnan;
+    a.u[c] = st ? (is_moving_init(t) ? a.U : T(0)) : qnan;
+    a.u[a.n3 + c] = st ? T(0) : qnan;
+    a.u[2 * a.n3 + c] = st ? T(0) : qnan;
+    S* pa = static_cast<S*>(a.a);
+    S* pb = static_cast<S*>(a.b);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int32_t dx = x + ex_of(i), dy = y + ey_of(i), dz = z + ez_of(i);
+        T v = qnan;
+        if (dx >= 0 && dy >= 0 && dz >= 0 && dx < n && dy < n && dz < n) {
+            const int32_t td = cell_type(dx, dy, dz, n);
+            if (!is_wall(td)) {
+                // kernels.cl:292-308 with rho = 1, u = (U or 0, 0, 0)
+                const T ux = is_moving_init(td) ? a.U : T(0);
+                const T u2 = ((ux * ux) + (T(0) * T(0))) + (T(0) * T(0));
+                v = feq<T>(i, T(1), ux, T(0), T(0), u2);
+            }
+        }
+        const int64_t s = slot(c, i, a.stride);
+        codec.store(pa + s, w_of<T>(i), v);
+        codec.store(pb + s, w_of<T>(i), v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Push view for lbm_read_f: out[slot(d, i)] = P[slot(d - e_i, i)] decoded,
+// NaN where d - e_i is outside the lattice.
+// ---------------------------------------------------------------------------
+template <class T> struct ViewArgs {
+    const void* p;
+    T* out;
+    int64_t n3;
+    int64_t stride;
+    int32_t n;
+    int32_t log2n;
+    float scale, inv_scale;
+};
+
+template <class T, int KIND>
+__global__ __launch_bounds__(256) void lbm_view_kernel(ViewArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= a.n3) return;
+    const int32_t x = int32_t(c & (n - 1));
+    const int32_t y = int32_t((c >> a.log2n) & (n - 1));
+    const int32_t z = int32_t(c >> (2 * a.log2n));
+    const S* p = static_cast<const S*>(a.p);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int32_t sx = x - ex_of(i), sy = y - ey_of(i), sz = z - ez_of(i);
+        T v = __builtin_nan("");
+        if (sx >= 0 && sy >= 0 && sz >= 0 && sx < n && sy < n && sz < n) {
+            const int64_t s = int64_t(sx) + int64_t(n) * (int64_t(sy) + int64_t(n) * sz);
+            v = codec.load(p + slot(s, i, a.stride), w_of<T>(i));
+        }
+        a.out[slot(c, i, a.stride)] = v;
+    }
+}
+
+// Final reduction of the per-block (mass, energy) partials, in a fixed order:
+// thread j sums blocks j, j + 1024, ... then a tree over the 1024 threads.
+__global__ __launch_bounds__(1024) void lbm_reduce_partials(const double* partials, int64_t nblocks, double* out) {
+    __shared__ double sm[1024];
+    __shared__ double se[1024];
+    double m = 0.0, e = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 1024) {
+        m += partials[2 * b];
+        e += partials[2 * b + 1];
+    }
+    sm[threadIdx.x] = m;
+    se[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+            sm[threadIdx.x] += sm[threadIdx.x + s];
+            se[threadIdx.x] += se[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sm[0];
+        out[1] = se[0];
+    }
+}
+
+}  // namespace lbm

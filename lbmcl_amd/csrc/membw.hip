@@ -1,0 +1,66 @@
+// membw.hip -- a plain device-to-device copy kernel of known size.
+//
+// Used by bench.py for two things: the sustained copy bandwidth of the GPU it
+// runs on (read + write bytes / kernel time, HIP events on the kernel's own
+// stream), and as the calibration point of the rocprofv3 FETCH_SIZE /
+// WRITE_SIZE counters (a copy of B bytes must fetch and write B bytes).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace {
+
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   int64_t n) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4 v = src[i];
+        __builtin_nontemporal_store(v.x, &dst[i].x);
+        __builtin_nontemporal_store(v.y, &dst[i].y);
+        __builtin_nontemporal_store(v.z, &dst[i].z);
+        __builtin_nontemporal_store(v.w, &dst[i].w);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Copy `bytes` (a multiple of 16) `reps` times between two fresh device
+// buffers; on success ms[r] holds the duration of repetition r.
+int membw_copy(int device, int64_t bytes, int reps, double* ms) {
+    if (bytes <= 0 || (bytes % 16) != 0 || reps <= 0 || !ms) return -1;
+    if (hipSetDevice(device) != hipSuccess) return -2;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, size_t(bytes)) != hipSuccess || hipMalloc(&b, size_t(bytes)) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = -3;
+    }
+    if (!rc && hipMemsetAsync(a, 1, size_t(bytes), s) != hipSuccess) rc = -4;
+    const int64_t n = bytes / 16;
+    const int blocks = 256 * 64;  // persistent-style grid, grid-stride loop
+    for (int r = 0; !rc && r < reps; ++r) {
+        if (hipEventRecord(e0, s) != hipSuccess) rc = -4;
+        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4*>(a),
+                           static_cast<uint4*>(b), n);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess)
+            rc = -4;
+        float t = 0.f;
+        if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = -4;
+        ms[r] = double(t);
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
+}  // extern "C"

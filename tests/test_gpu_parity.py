@@ -1,0 +1,203 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU oracle.
+
+The HIP path and the oracle evaluate the same expressions in the same order
+without contraction, so the expected agreement is bit for bit: rho, u, the full
+CSoA lattice in the reference's push view, and the raw stored state.  The
+north-star tolerances (1e-5 relative in FP32, 1e-12 in FP64) are asserted as
+well, so a failure message distinguishes "bits differ" from "out of
+tolerance".
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbmcl_amd import capi
+from lbmcl_amd.verify import metrics
+from lbmcl_amd.vti import read_vti
+from tests.util import max_rel, mismatch_count, nan_equal, window, window_v
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 1e-5, "f64": 1e-12}
+STORE = {"native": O.STORE_NATIVE, "fp16": O.STORE_FP16, "bf16": O.STORE_BF16}
+
+
+def _pair(n, stride, precision, storage="native", kernel=capi.LBM_KERNEL_AUTO, lws=(0, 0, 0),
+          viscosity=0.0089, velocity=0.05):
+    g = capi.Context(n, viscosity=viscosity, velocity=velocity, stride=stride, precision=precision,
+                     storage=storage, kernel=kernel, lws=lws)
+    o = O.PullSim(n, stride=stride, viscosity=viscosity, velocity=velocity, precision=precision,
+                  storage=STORE[storage])
+    g.initialize()
+    o.initialize()
+    return g, o
+
+
+def _assert_macro_equal(g, o, n, precision, label):
+    rg, ug = g.read_rho(), g.read_u()
+    ro, uo = o.macro()
+    rel_r = max_rel(rg, ro)
+    rel_u = max_rel(ug, uo.reshape(-1))
+    print(f"[{label}] max rel drho={rel_r:.3e} du={rel_u:.3e}", flush=True)
+    assert rel_r <= TOL[precision] and rel_u <= TOL[precision], label
+    assert nan_equal(rg, ro), f"{label}: rho bits differ in {mismatch_count(rg, ro)} cells"
+    assert nan_equal(ug, uo.reshape(-1)), f"{label}: u bits differ in {mismatch_count(ug, uo.reshape(-1))} entries"
+
+
+CASES = [
+    # n, stride, precision, kernel, steps
+    (8, 8, "f32", capi.LBM_KERNEL_AUTO, 10),
+    (8, 1, "f64", capi.LBM_KERNEL_AUTO, 10),
+    (16, 4096, "f32", capi.LBM_KERNEL_GENERIC, 16),
+    (32, 32, "f64", capi.LBM_KERNEL_GENERIC, 24),
+    (64, 64, "f32", capi.LBM_KERNEL_ROW, 12),
+    (64, 4, "f32", capi.LBM_KERNEL_ROW, 12),
+    (64, 1, "f64", capi.LBM_KERNEL_ROW, 12),
+    (64, 64 ** 3, "f64", capi.LBM_KERNEL_ROW, 12),
+    (64, 64, "f32", capi.LBM_KERNEL_GENERIC, 12),
+    (128, 64, "f32", capi.LBM_KERNEL_AUTO, 8),
+    (128, 64, "f64", capi.LBM_KERNEL_AUTO, 8),
+    (256, 256 ** 3, "f32", capi.LBM_KERNEL_ROW, 4),
+]
+
+
+@pytest.mark.parametrize("n,stride,precision,kernel,steps", CASES)
+def test_native_matches_oracle(n, stride, precision, kernel, steps):
+    label = f"n={n} s={stride} {precision} k={kernel}"
+    print(label, flush=True)
+    g, o = _pair(n, stride, precision, kernel=kernel)
+    with g:
+        _assert_macro_equal(g, o, n, precision, label + " init")
+        assert np.array_equal(g.read_map(), o.map)
+        for it in range(1, steps + 1):
+            g.compute(it, update_macro=True)
+            o.step(update_macro=True)
+            if it in (1, 2, steps):
+                _assert_macro_equal(g, o, n, precision, f"{label} it={it}")
+        # full lattice, both buffers, in the reference's push view
+        for buf, ob in ((0, o.A), (1, o.B)):
+            fg, fo = g.read_f(buf), o.view(ob)
+            assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
+            sg = g.read_state(buf)
+            assert nan_equal(sg, ob), f"{label}: stored state {buf} differs"
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+@pytest.mark.parametrize("n,kernel", [(16, capi.LBM_KERNEL_GENERIC), (64, capi.LBM_KERNEL_ROW)])
+def test_shifted_storage_matches_oracle(storage, n, kernel):
+    label = f"{storage} n={n} k={kernel}"
+    g, o = _pair(n, n, "f32", storage=storage, kernel=kernel)
+    with g:
+        for it in range(1, 13):
+            g.compute(it, update_macro=True)
+            o.step(update_macro=True)
+        _assert_macro_equal(g, o, n, "f32", label)
+        for buf, ob in ((0, o.A), (1, o.B)):
+            fg, fo = g.read_f(buf), o.view(ob)
+            assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
+
+
+@pytest.mark.parametrize("n,precision,kernel", [(16, "f32", capi.LBM_KERNEL_GENERIC), (64, "f64", capi.LBM_KERNEL_ROW),
+                                                (128, "f32", capi.LBM_KERNEL_ROW)])
+def test_diagnostics(n, precision, kernel):
+    g, o = _pair(n, n, precision, kernel=kernel)
+    with g:
+        for it in range(1, 7):
+            g.compute(it, update_macro=True, diagnostics=(it % 3 == 0))
+            o.step(update_macro=True)
+            if it % 3 == 0:
+                mg, eg = g.read_diagnostics()
+                mo, eo = o.diag
+                # same per-cell FP64 terms summed in a different order
+                assert abs(mg - mo) <= 1e-12 * abs(mo), (mg, mo)
+                assert abs(eg - eo) <= 1e-10 * abs(eo), (eg, eo)
+                ref = O.diagnostics(o.p, o.rho, o.u, precision)
+                assert abs(mg - ref[0]) <= 1e-12 * abs(ref[0])
+
+
+@pytest.mark.parametrize("lws", [(8, 8, 8), (32, 1, 1), (7, 3, 1), (1, 1, 1)])
+def test_block_shape_invariance(lws):
+    """The result does not depend on the work-group size (the reference's -w)."""
+    g, o = _pair(16, 16, "f32", kernel=capi.LBM_KERNEL_GENERIC, lws=lws)
+    with g:
+        for it in range(1, 6):
+            g.compute(it, update_macro=True, diagnostics=(it == 5))
+            o.step(update_macro=True)
+        _assert_macro_equal(g, o, 16, "f32", f"lws={lws}")
+        mg, _ = g.read_diagnostics()
+        assert abs(mg - o.diag[0]) <= 1e-12 * abs(o.diag[0])
+
+
+def _fixture_run(golden_dir, n, iterations, every, thresholds):
+    g = capi.Context(n, stride=n, precision="f32")
+    w = len(str(iterations))
+    with g:
+        g.initialize()
+        rows = []
+        for it in range(0, iterations + 1):
+            if it > 0:
+                g.compute(it, update_macro=(it % every == 0))
+            if it % every == 0:
+                t = read_vti(os.path.join(golden_dir, "target_results", str(n), f"ldc.0.{it:0{w}d}.vti"))
+                rows.append(metrics(t["rho"], t["v"], window(g.read_rho(), n), window_v(g.read_u(), n)))
+    worst = np.max(np.array(rows), axis=0)
+    print(f"n={n}: worst MSE_rho={worst[0]:.3e} MSE_u={worst[1]:.3e} MAE_rho={worst[2]:.3e} MAE_u={worst[3]:.3e}")
+    for v, lim, name in zip(worst, thresholds, ("mse_rho", "mse_u", "mae_rho", "mae_u")):
+        assert v <= lim, f"{name} {v} > {lim}"
+
+
+def test_fixture_8(golden_dir):
+    """make test8 (Makefile:73-78): 8^3, 10 iterations, every 1, against Sailfish."""
+    _fixture_run(golden_dir, 8, 10, 1, (1e-13, 5e-15, 1e-6, 4e-7))
+
+
+def test_fixture_32(golden_dir):
+    """make test32 (Makefile:81-86): 32^3, 500 iterations, every 20, against Sailfish."""
+    _fixture_run(golden_dir, 32, 500, 20, (2e-12, 3e-14, 4e-6, 1.2e-6))
+
+
+def test_events_and_timing():
+    with capi.Context(64, stride=64, profile=True) as g:
+        g.initialize()
+        g.compute_range(1, 5, every=5)
+        g.read_rho()
+        ev = g.events()
+        names = [e[0] for e in ev]
+        assert names[0] == "initialize" and names.count("compute") == 5 and names[-1] == "read_rho"
+        kt, tt = g.kernels_time_ms(), g.total_time_ms()
+        assert 0 < kt <= tt
+        assert abs(kt - sum(ms for name, ms in ev if name == "compute")) < 1e-9
+
+
+def test_state_roundtrip():
+    """lbm_write_state restores what lbm_read_state returned."""
+    with capi.Context(32, stride=32) as g:
+        g.initialize()
+        for it in range(1, 5):
+            g.compute(it)
+        s0, s1 = g.read_state(0), g.read_state(1)
+        g.compute(5, update_macro=True)
+        r5 = g.read_rho()
+        g.write_state(0, s0)
+        g.write_state(1, s1)
+        g.compute(5, update_macro=True)
+        assert nan_equal(g.read_rho(), r5)
+
+
+def test_full_size_512_f32():
+    """512^3 FP32, the bandwidth-scale configuration: 2 steps bit-identical to the oracle."""
+    n = 512
+    g = capi.Context(n, stride=n ** 3, precision="f32")
+    o = O.PullSim(n, stride=n ** 3, precision="f32")
+    with g:
+        g.initialize()
+        o.initialize()
+        for it in (1, 2):
+            g.compute(it, update_macro=True)
+            o.step(update_macro=True)
+        rg, ro = g.read_rho(), o.rho
+        assert nan_equal(rg, ro), mismatch_count(rg, ro)
+        ug = g.read_u()
+        assert nan_equal(ug, o.u), mismatch_count(ug, o.u)
