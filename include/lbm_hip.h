@@ -53,10 +53,17 @@ extern "C" {
 #define LBM_STORE_FP16_SHIFT 1
 #define LBM_STORE_BF16_SHIFT 2
 
-/* kernel selection (LBM_KERNEL_AUTO picks the row kernel when dim >= 64) */
+/* kernel selection.  The vector row kernels give each lane 16 (VEC) or 8
+ * (VEC8) bytes of every population: V = lane bytes / bytes per stored
+ * population consecutive cells; they need stride >= V and dim / V >= 64.
+ * LBM_KERNEL_AUTO picks VEC for 8-byte populations and VEC8 for 4-byte and
+ * 16-bit ones when possible, else the row kernel when dim >= 64, else the generic
+ * kernel.  All kernels compute identical results. */
 #define LBM_KERNEL_AUTO 0
 #define LBM_KERNEL_GENERIC 1
 #define LBM_KERNEL_ROW 2
+#define LBM_KERNEL_VEC 3
+#define LBM_KERNEL_VEC8 4
 
 /* event kinds, named like the reference's events (lbmcl.hpp:26-31) */
 #define LBM_EVENT_INITIALIZE 0
@@ -76,7 +83,7 @@ typedef struct lbm_config {
     double viscosity;    /* kinematic viscosity nu; tau = 3 nu + 1/2 (kernels.cl:61-62) */
     double velocity;     /* lid x-velocity U (kernels.cl:57) */
     int32_t lws[3];      /* work-group size of the generic kernel (product <= 1024); 0 = default */
-    int32_t optimize;    /* -o: reserved, the kernels always compute without contraction */
+    int32_t optimize;    /* -o: fused collision (see lbm_compute); 16-bit storage always uses it */
     int32_t kernel;      /* LBM_KERNEL_* */
     int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
     int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernel) */
@@ -111,7 +118,13 @@ int lbm_initialize(lbm_ctx* ctx);
  * update_macro stores rho and u of FLUID/MOVING cells (kernels.cl:383-388).
  * diagnostics != 0 also reduces the mass sum(rho) and kinetic energy
  * sum(rho |u|^2 / 2) over FLUID/MOVING cells in FP64; read them with
- * lbm_read_diagnostics (no counterpart in the reference). */
+ * lbm_read_diagnostics (no counterpart in the reference).
+ * Arithmetic: by default each expression of kernels.cl:343-418 is evaluated
+ * operation by operation without contraction.  With optimize != 0 (the -o
+ * option, the reference's -cl-fast-relaxed-math) and always for 16-bit
+ * storage, the BGK update uses a fused form, f' = (1 - 1/tau) f + (rho w_i / tau) T_i
+ * with T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u, evaluated with
+ * fused multiply-adds; rho and u are unchanged, results differ by rounding. */
 int lbm_compute(lbm_ctx* ctx, int64_t iteration, int32_t update_macro, int32_t diagnostics);
 
 /* Enqueue iterations first .. first+count-1; update_macro is set on those

@@ -151,10 +151,49 @@ int dev_alloc(void** p, size_t bytes, const char* what) {
     return LBM_OK;
 }
 
-bool use_row_kernel(const lbm_ctx* c) {
-    if (c->cfg.kernel == LBM_KERNEL_GENERIC) return false;
-    return c->n >= 64;
+enum KernelKind { K_GENERIC, K_ROW, K_VEC, K_VEC8 };
+
+// bytes per lane and population of the vector kernels
+int64_t lane_bytes(KernelKind k) { return k == K_VEC8 ? 8 : 16; }
+
+bool vec_possible(const lbm_ctx* c, KernelKind k) {
+    const int64_t v = lane_bytes(k) / int64_t(c->ssize);  // cells per lane
+    return v >= 1 && c->stride >= v && c->n / v >= 64;
 }
+
+// Preferred vector width per storage type (measured on MI355X at 512^3):
+// 16-byte lanes for FP64 (2 cells per lane), 8-byte lanes for FP32 and 16-bit
+// storage, whose 16-byte forms need too many registers for the occupancy the
+// kernel wants.
+KernelKind preferred_vec(const lbm_ctx* c) { return c->ssize == 8 ? K_VEC : K_VEC8; }
+
+KernelKind kernel_kind(const lbm_ctx* c) {
+    switch (c->cfg.kernel) {
+        case LBM_KERNEL_GENERIC: return K_GENERIC;
+        case LBM_KERNEL_ROW: return K_ROW;
+        case LBM_KERNEL_VEC: return K_VEC;
+        case LBM_KERNEL_VEC8: return K_VEC8;
+        default: {
+            const KernelKind p = preferred_vec(c);
+            if (vec_possible(c, p)) return p;
+            if (vec_possible(c, K_VEC8)) return K_VEC8;
+            return c->n >= 64 ? K_ROW : K_GENERIC;
+        }
+    }
+}
+
+// cells per lane of the selected kernel
+int64_t vec_width(const lbm_ctx* c) {
+    const KernelKind k = kernel_kind(c);
+    return (k == K_VEC || k == K_VEC8) ? lane_bytes(k) / int64_t(c->ssize) : 1;
+}
+
+// threads per block and blocks per row of the row kernels
+int64_t row_block(const lbm_ctx* c) {
+    const int64_t per_row = c->n / vec_width(c);
+    return per_row < 256 ? per_row : 256;
+}
+int64_t row_gx(const lbm_ctx* c) { return (c->n / vec_width(c)) / row_block(c); }
 
 int generic_block(const lbm_ctx* c) {
     int64_t b = 1;
@@ -165,16 +204,19 @@ int generic_block(const lbm_ctx* c) {
 
 // Number of blocks of one compute launch (for the diagnostics partials).
 int64_t compute_blocks(const lbm_ctx* c) {
-    if (use_row_kernel(c)) {
-        const int64_t bx = c->n < 256 ? c->n : 256;
-        return (c->n / bx) * int64_t(c->n - 2) * int64_t(c->n - 2);
-    }
+    const KernelKind k = kernel_kind(c);
+    if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n);  // all rows
+    if (k == K_ROW) return row_gx(c) * int64_t(c->n - 2) * int64_t(c->n - 2);         // interior rows
     const int64_t b = generic_block(c);
     return (c->n3 + b - 1) / b;
 }
 
-template <class T, int KIND>
-int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
+// The fused collision is used for 16-bit storage always and for native
+// storage when fast math (-o) is requested.
+bool use_fast(const lbm_ctx* c) { return c->cfg.storage != LBM_STORE_NATIVE || c->cfg.optimize != 0; }
+
+template <class T, int KIND, bool FAST>
+int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
     lbm::StepArgs<T> a{};
     a.src = src;
     a.dst = dst;
@@ -190,22 +232,48 @@ int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int
     a.xcd_remap = c->cfg.xcd_remap;
     a.k.inv_tau = T(c->inv_tau);
     a.k.U = T(c->U);
+    a.k.om1 = T(1) - T(c->inv_tau);
     a.scale = float(c->scale);
     a.inv_scale = float(1.0 / c->scale);
-    if (use_row_kernel(c)) {
-        const int bx = c->n < 256 ? c->n : 256;
-        a.gx = c->n / bx;
-        const int64_t nb = int64_t(a.gx) * int64_t(c->n - 2) * int64_t(c->n - 2);
-        hipLaunchKernelGGL((lbm::lbm_collide_stream_row<T, KIND>), dim3(unsigned(nb)), dim3(bx), 0, c->stream, a);
+    {
+        // lid-layer initial populations, same expression as the initialize kernel
+        const T U = T(c->U);
+        const T u2 = ((U * U) + (T(0) * T(0))) + (T(0) * T(0));
+        for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq<T>(i, T(1), U, T(0), T(0), u2);
+    }
+    const KernelKind kk = kernel_kind(c);
+    if (kk != K_GENERIC) {
+        a.gx = int32_t(row_gx(c));
+        a.log2gx = ilog2(a.gx);
+        const int64_t nb = compute_blocks(c);
+        const unsigned bx = unsigned(row_block(c));
+        if (kk == K_VEC)
+            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 16, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
+                               a);
+        else if (kk == K_VEC8)
+            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 8, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
+                               a);
+        else
+            hipLaunchKernelGGL((lbm::lbm_collide_stream_row<T, KIND, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream, a);
     } else {
         const int b = generic_block(c);
         const int64_t nb = (c->n3 + b - 1) / b;
         a.gx = 0;
-        hipLaunchKernelGGL((lbm::lbm_collide_stream_generic<T, KIND>), dim3(unsigned(nb)), dim3(b), 0, c->stream,
+        hipLaunchKernelGGL((lbm::lbm_collide_stream_generic<T, KIND, FAST>), dim3(unsigned(nb)), dim3(b), 0, c->stream,
                            a);
     }
     LBM_HIP_CHECK(hipGetLastError(), "compute launch");
     return LBM_OK;
+}
+
+template <class T, int KIND>
+int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
+    if constexpr (KIND != lbm::ST_NATIVE) {
+        return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
+    } else {
+        if (use_fast(c)) return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
+        return launch_compute_impl<T, KIND, false>(c, src, dst, update_macro, diag);
+    }
 }
 
 template <class T, int KIND>
@@ -323,9 +391,17 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         return fail(LBM_ERR_INVALID, "unknown storage");
     if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
         return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
-    if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW)
+    if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
+        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8)
         return fail(LBM_ERR_INVALID, "unknown kernel selection");
     if (k.kernel == LBM_KERNEL_ROW && k.dim < 64) return fail(LBM_ERR_INVALID, "the row kernel needs dim >= 64");
+    if (k.kernel == LBM_KERNEL_VEC || k.kernel == LBM_KERNEL_VEC8) {
+        const int64_t lb = (k.kernel == LBM_KERNEL_VEC) ? 16 : 8;
+        const int64_t v = lb / ((k.storage != LBM_STORE_NATIVE) ? 2 : (k.precision == LBM_F64 ? 8 : 4));
+        if (k.stride < v || k.dim / v < 64)
+            return fail(LBM_ERR_INVALID, "the vector kernel needs stride >= V and dim / V >= 64 (V = lane bytes / "
+                                         "bytes per stored population)");
+    }
     if (!(k.viscosity >= 0.0) || !std::isfinite(k.viscosity)) return fail(LBM_ERR_INVALID, "invalid viscosity");
     if (!std::isfinite(k.velocity)) return fail(LBM_ERR_INVALID, "invalid velocity");
     int64_t lws = 1;

@@ -37,33 +37,38 @@ template <class T, int KIND> struct Codec;
 template <class T> struct Codec<T, ST_NATIVE> {
     using S = T;
     float scale, inv_scale;
-    __device__ __forceinline__ T load(const S* p, T) const { return *p; }
-    __device__ __forceinline__ void store(S* p, T, T v) const { __builtin_nontemporal_store(v, p); }
+    __device__ __forceinline__ T decode(S s, T) const { return s; }
+    __device__ __forceinline__ S encode(T v, T) const { return v; }
+    __device__ __forceinline__ T load(const S* p, T w) const { return decode(*p, w); }
+    __device__ __forceinline__ void store(S* p, T w, T v) const { __builtin_nontemporal_store(encode(v, w), p); }
 };
 
 template <> struct Codec<float, ST_FP16> {
     using S = _Float16;
     float scale, inv_scale;
-    __device__ __forceinline__ float load(const S* p, float w) const { return w + float(*p) * inv_scale; }
-    __device__ __forceinline__ void store(S* p, float w, float v) const {
-        const _Float16 h = _Float16((v - w) * scale);
-        __builtin_nontemporal_store(h, p);
+    // h * inv_scale is exact, so the fma equals w + h * inv_scale; v * scale - w * scale
+    // is (v - w) * scale exactly before its single rounding (scale is a power of two)
+    __device__ __forceinline__ float decode(S s, float w) const { return __builtin_fmaf(float(s), inv_scale, w); }
+    __device__ __forceinline__ S encode(float v, float w) const {
+        return _Float16(__builtin_fmaf(v, scale, -(w * scale)));
     }
+    __device__ __forceinline__ float load(const S* p, float w) const { return decode(*p, w); }
+    __device__ __forceinline__ void store(S* p, float w, float v) const { __builtin_nontemporal_store(encode(v, w), p); }
 };
 
 template <> struct Codec<float, ST_BF16> {
     using S = uint16_t;
     float scale, inv_scale;
-    __device__ __forceinline__ float load(const S* p, float w) const {
-        return w + __uint_as_float(uint32_t(*p) << 16) * inv_scale;
+    __device__ __forceinline__ float decode(S s, float w) const {
+        return __builtin_fmaf(__uint_as_float(uint32_t(s) << 16), inv_scale, w);
     }
-    __device__ __forceinline__ void store(S* p, float w, float v) const {
-        // round to nearest even; NaN stays a quiet NaN
-        const uint32_t x = __float_as_uint((v - w) * scale);
-        const uint16_t h = ((x & 0x7fffffffu) > 0x7f800000u) ? uint16_t((x >> 16) | 0x40u)
-                                                               : uint16_t((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
-        __builtin_nontemporal_store(h, p);
+    __device__ __forceinline__ S encode(float v, float w) const {
+        // bfloat16 conversion rounds to nearest even (a NaN stays a NaN)
+        const __bf16 h = __bf16(__builtin_fmaf(v, scale, -(w * scale)));
+        return __builtin_bit_cast(uint16_t, h);
     }
+    __device__ __forceinline__ float load(const S* p, float w) const { return decode(*p, w); }
+    __device__ __forceinline__ void store(S* p, float w, float v) const { __builtin_nontemporal_store(encode(v, w), p); }
 };
 
 // ---------------------------------------------------------------------------
@@ -91,19 +96,20 @@ __host__ __device__ constexpr int ez_of(int i) {
            : (i == 5 || i == 11 || i == 12 || i == 13 || i == 14) ? -1
                                                                    : 0;
 }
-template <class T> __device__ __forceinline__ T w_of(int i) {
+template <class T> __host__ __device__ __forceinline__ T w_of(int i) {
     return i == 0 ? W<T>::w0 : (i <= 6 ? W<T>::w1 : W<T>::w7);
 }
 
 template <class T> struct StepConsts {
     T inv_tau;  // 1 / (3 nu + 1/2) folded in T (kernels.cl:61-62)
     T U;        // lid velocity literal (kernels.cl:57)
+    T om1;      // 1 - inv_tau, for the fused collision
 };
 
 // e_i . u with the reference's ((ux ex + uy ey) + uz ez) evaluated exactly:
 // products with 0 contribute +-0 and products with -1 are negations, so the
 // expression reduces to the signed sum of the non-zero components in order.
-template <class T> __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
+template <class T> __host__ __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
     switch (i) {
         case 0: return T(0);
         case 1: return ux;
@@ -128,7 +134,7 @@ template <class T> __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
 }
 
 // (rho w_i) (1 + 3 eu + 4.5 eu eu - 1.5 u2), left to right (kernels.cl:397, 416).
-template <class T> __device__ __forceinline__ T feq(int i, T rho, T ux, T uy, T uz, T u2) {
+template <class T> __host__ __device__ __forceinline__ T feq(int i, T rho, T ux, T uy, T uz, T u2) {
     const T eu = edot<T>(i, ux, uy, uz);
     return (rho * w_of<T>(i)) * (((T(1) + (T(3) * eu)) + ((T(4.5) * eu) * eu)) - (T(1.5) * u2));
 }
@@ -186,6 +192,88 @@ __device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& 
         tmp = f[14]; f[14] = f[16]; f[16] = tmp;
     }
     // CORNER: pure propagation (kernels.cl:254-260 with no BC and no collision).
+}
+
+// Fused form of the same local step, used by the 16-bit storage forms and by
+// the native forms when fast math is requested (-o).  Moments, macroscopic
+// values and boundary handling are those of collide(); the BGK update is
+// rewritten as f' = (1 - w) f + (w rho w_i) T_i with
+//   T_i = K + e (3 + 4.5 e),  K = 1 - 1.5 u2,  e = e_i . u,
+// built with fused multiply-adds and shared between opposite populations
+// (T for -e is A - 3e with A = 4.5 e e + K).  Every operation is a single IEEE
+// rounding, so the CPU oracle reproduces it with std::fma.
+template <class T> __device__ __forceinline__ T fmaT(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+template <class T>
+__device__ __forceinline__ void collide_fast(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+                                             const StepConsts<T>& k) {
+    const bool moving = is_moving(t);
+    if (moving) {
+        f[5] = f[6];
+        f[11] = f[17];
+        f[12] = f[18];
+        f[13] = f[15];
+        f[14] = f[16];
+    }
+    rho = f[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) rho = rho + f[i];
+    if (moving) {
+        ux = k.U;
+        uy = T(0);
+        uz = T(0);
+    } else {
+        const T jx = ((((f[1] + f[7]) + f[10]) + f[11]) + f[15]) - ((((f[3] + f[8]) + f[9]) + f[13]) + f[17]);
+        const T jy = ((((f[2] + f[7]) + f[8]) + f[12]) + f[16]) - ((((f[4] + f[9]) + f[10]) + f[14]) + f[18]);
+        const T jz = ((((f[6] + f[15]) + f[16]) + f[17]) + f[18]) - ((((f[5] + f[11]) + f[12]) + f[13]) + f[14]);
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = jz / rho;
+    }
+    if (!moving && t != FLUID) {
+        if (is_bounceback(t)) {
+            T tmp;
+            tmp = f[1]; f[1] = f[3]; f[3] = tmp;
+            tmp = f[2]; f[2] = f[4]; f[4] = tmp;
+            tmp = f[5]; f[5] = f[6]; f[6] = tmp;
+            tmp = f[7]; f[7] = f[9]; f[9] = tmp;
+            tmp = f[8]; f[8] = f[10]; f[10] = tmp;
+            tmp = f[11]; f[11] = f[17]; f[17] = tmp;
+            tmp = f[12]; f[12] = f[18]; f[18] = tmp;
+            tmp = f[13]; f[13] = f[15]; f[15] = tmp;
+            tmp = f[14]; f[14] = f[16]; f[16] = tmp;
+        }
+        return;
+    }
+    const T u2 = fmaT<T>(uz, uz, fmaT<T>(uy, uy, ux * ux));
+    const T K = fmaT<T>(T(-1.5), u2, T(1));
+    // moving: f_i = (rho w_i) T_i;  fluid: f_i = (inv_tau rho w_i) T_i + om1 f_i
+    const T s = moving ? rho : k.inv_tau * rho;
+    const T c0 = s * W<T>::w0, c1 = s * W<T>::w1, c7 = s * W<T>::w7;
+    const T keep = moving ? T(0) : k.om1;
+    constexpr int PI[9] = {1, 2, 6, 7, 10, 15, 16, 11, 12};    // e_i
+    constexpr int PO[9] = {3, 4, 5, 9, 8, 13, 14, 17, 18};     // -e_i
+    f[0] = fmaT<T>(c0, K, keep * f[0]);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int i = PI[p], o = PO[p];
+        const T e = edot<T>(i, ux, uy, uz);
+        const T A = fmaT<T>(T(4.5) * e, e, K);
+        const T B = T(3) * e;
+        const T c = (i <= 6) ? c1 : c7;
+        f[i] = fmaT<T>(c, A + B, keep * f[i]);
+        f[o] = fmaT<T>(c, A - B, keep * f[o]);
+    }
+}
+
+template <class T, bool FAST>
+__device__ __forceinline__ void collide_any(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+                                            const StepConsts<T>& k) {
+    if constexpr (FAST) collide_fast<T>(t, f, rho, ux, uy, uz, k);
+    else collide<T>(t, f, rho, ux, uy, uz, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -257,9 +345,11 @@ template <class T> struct StepArgs {
     int32_t update_macro;
     int32_t diagnostics;
     int32_t xcd_remap;
-    int32_t gx;           // row kernel: blocks per row
+    int32_t gx;           // row kernels: blocks per row
+    int32_t log2gx;       // vector kernel: log2(gx)
     StepConsts<T> k;
     float scale, inv_scale;
+    T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
 };
 
 // ---------------------------------------------------------------------------
@@ -279,7 +369,7 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb, i
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <class T, int KIND>
+template <class T, int KIND, bool FAST>
 __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
     using C = Codec<T, KIND>;
     using S = typename C::S;
@@ -318,7 +408,7 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
             const uint32_t off = ex == 1 ? v_m : (ex == -1 ? v_p : v_0);
             f[i] = codec.load(src + base + off, w_of<T>(i));
         }
-        collide<T>(t, f, rho, ux, uy, uz, a.k);
+        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
         const int64_t c0 = int64_t(n) * r;
         const int64_t obase = 19 * c0 - 18 * (c0 & smask);
 #pragma unroll
@@ -344,10 +434,221 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Vector row kernel.  Each lane owns V = LB / sizeof(S) consecutive cells of a
+// row (LB = 16 or 8 bytes: 4 or 2 cells in FP32, 2 or 1 in FP64, 8 or 4 with
+// 16-bit storage), so every population moves as one LB-byte access per lane: an aligned vector load of the lane's own
+// slots, and for populations with e_x = +-1 the one element that crosses into
+// the neighbouring lane comes from that lane by a cross-lane shuffle (a
+// wavefront-uniform load at the two wavefront ends).  Stores are whole aligned
+// vectors: the two wall cells of a row (x = 0, N-1) get their initial pull-form
+// values back, which is what they hold already, so wall slots keep P0.
+// Requirements: stride >= V (V consecutive cells are contiguous and LB-byte
+// aligned for every population) and N / V >= 64 (a wavefront lies in one row).
+// ---------------------------------------------------------------------------
+// Cross-lane moves by one lane within the wavefront (DPP wave_shr:1 /
+// wave_shl:1).  The lane at the wavefront's end, which has no source lane,
+// receives `edge`.
+__device__ __forceinline__ uint32_t dpp_from_below(uint32_t v, uint32_t edge) {  // lane l gets lane l-1
+    return uint32_t(__builtin_amdgcn_update_dpp(int(edge), int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t dpp_from_above(uint32_t v, uint32_t edge) {  // lane l gets lane l+1
+    return uint32_t(__builtin_amdgcn_update_dpp(int(edge), int(v), 0x130, 0xf, 0xf, false));
+}
+template <class S> __device__ __forceinline__ S lane_shift(S v, S edge, bool from_below) {
+    if constexpr (sizeof(S) == 8) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v), e = __builtin_bit_cast(uint64_t, edge);
+        const uint32_t lo = from_below ? dpp_from_below(uint32_t(b), uint32_t(e)) : dpp_from_above(uint32_t(b), uint32_t(e));
+        const uint32_t hi = from_below ? dpp_from_below(uint32_t(b >> 32), uint32_t(e >> 32))
+                                       : dpp_from_above(uint32_t(b >> 32), uint32_t(e >> 32));
+        return __builtin_bit_cast(S, (uint64_t(hi) << 32) | lo);
+    } else if constexpr (sizeof(S) == 4) {
+        const uint32_t b = __builtin_bit_cast(uint32_t, v), e = __builtin_bit_cast(uint32_t, edge);
+        return __builtin_bit_cast(S, from_below ? dpp_from_below(b, e) : dpp_from_above(b, e));
+    } else {
+        const uint32_t b = __builtin_bit_cast(uint16_t, v), e = __builtin_bit_cast(uint16_t, edge);
+        return __builtin_bit_cast(S, uint16_t(from_below ? dpp_from_below(b, e) : dpp_from_above(b, e)));
+    }
+}
+
+// Initial pull-form value of population i at the wall cell x = 0 (side -1) or
+// x = N-1 (side +1) of row (y, z): F0 of the destination (x + e_x, y + e_y,
+// z + e_z), i.e. NaN unless the destination is a fluid-side cell.
+template <class T>
+__device__ __forceinline__ T wall_p0(int i, int side, int32_t y, int32_t z, int32_t n, const T* lid_eq) {
+    const int32_t dy = y + ey_of(i), dz = z + ez_of(i);
+    const bool into = (side < 0) ? (ex_of(i) == 1) : (ex_of(i) == -1);
+    if (!into || dy < 1 || dy > n - 2 || dz < 1 || dz > n - 2) return T(__builtin_nan(""));
+    return dz == n - 2 ? lid_eq[i] : w_of<T>(i);
+}
+
+// Occupancy target of the vector kernel: the waves per SIMD its register
+// budget is fitted to (512 VGPRs per SIMD lane / waves).
+template <class T, int KIND, int LB> struct VecWaves {
+    static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? 4 : 3);
+};
+
+template <class T, int KIND, int LB, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VecWaves<T, KIND, LB>::value)))
+void lbm_collide_stream_vec(StepArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    constexpr int V = int(LB / sizeof(S));  // cells per lane
+    typedef S SV __attribute__((ext_vector_type(V)));
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    // grid: gx blocks per row, N^2 rows (wall rows exit at once); all powers of two
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    const uint32_t xb = lb & uint32_t(a.gx - 1);
+    const uint32_t row = lb >> a.log2gx;
+    const int32_t y = int32_t(row & uint32_t(n - 1));
+    const int32_t z = int32_t(row >> a.log2n);
+    if (y == 0 || y == n - 1 || z == 0 || z == n - 1) {
+        if (a.diagnostics && threadIdx.x == 0) {
+            a.partials[2 * lb] = 0.0;
+            a.partials[2 * lb + 1] = 0.0;
+        }
+        return;
+    }
+    const uint32_t tid = threadIdx.x;
+    const int32_t x0 = int32_t((xb * blockDim.x + tid) * V);
+    // first cell of this wavefront, made explicitly wavefront-uniform
+    const int32_t xw = __builtin_amdgcn_readfirstlane(int32_t((xb * blockDim.x + (tid & ~63u)) * V));
+    const int64_t r = int64_t(y) + int64_t(n) * z;
+    const int32_t rf = row_flags(y, z, n);
+    const int64_t smask = a.stride - 1;
+    const uint32_t smask32 = uint32_t(smask);
+    auto voff = [&](int32_t xx) -> uint32_t {  // v(x) = slot(N r + x) - slot(N r)
+        const uint32_t u = uint32_t(xx);
+        return 19u * u - 18u * (u & smask32);
+    };
+    auto src_row = [&](int i) -> const S* {  // slot 0 of the source row of population i
+        const int64_t c0 = int64_t(n) * (r - ey_of(i) - int64_t(n) * ez_of(i));
+        return static_cast<const S*>(a.src) + (19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride);
+    };
+    const uint32_t off0 = voff(x0);
+    const bool has_left = xw > 0;               // cell xw-1 exists (it is not needed when xw = 0)
+    const bool has_right = xw + 64 * V < n;     // cell xw+64V exists
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+
+    // all 19 vector loads first, then the wavefront-end elements, then the shifts
+    SV A[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) A[i] = *reinterpret_cast<const SV*>(src_row(i) + off0);
+    S edge[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        edge[i] = S(0);
+        if (ex_of(i) == 1 && has_left) edge[i] = src_row(i)[voff(xw - 1)];
+        if (ex_of(i) == -1 && has_right) edge[i] = src_row(i)[voff(xw + 64 * V)];
+    }
+    // gather: g[i][j] is population i arriving at cell x0 + j
+    S g[Q][V];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int ex = ex_of(i);
+        if (ex == 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) g[i][j] = A[i][j];
+        } else if (ex == 1) {  // source x - 1
+            g[i][0] = lane_shift<S>(A[i][V - 1], edge[i], true);
+#pragma unroll
+            for (int j = 1; j < V; ++j) g[i][j] = A[i][j - 1];
+        } else {  // source x + 1
+#pragma unroll
+            for (int j = 0; j < V - 1; ++j) g[i][j] = A[i][j + 1];
+            g[i][V - 1] = lane_shift<S>(A[i][0], edge[i], false);
+        }
+    }
+
+    T rho_v[V], ux_v[V], uy_v[V], uz_v[V];
+    double m = 0.0, e = 0.0;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int32_t t = finish_type(x_flags(x0 + j, n) | rf);
+        T f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = codec.decode(g[i][j], w_of<T>(i));
+        T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
+        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
+        const bool store = is_store_macro(t);
+        const T qnan = T(__builtin_nan(""));
+        rho_v[j] = store ? rho : qnan;
+        ux_v[j] = store ? ux : qnan;
+        uy_v[j] = store ? uy : qnan;
+        uz_v[j] = store ? uz : qnan;
+        if (a.diagnostics && store) {
+            double dm, de;
+            diag_terms<T>(rho, ux, uy, uz, dm, de);
+            m += dm;
+            e += de;
+        }
+        if (j == 0 || j == V - 1) {
+            // the row's wall cells keep their initial values
+            const bool wl = (j == 0) && (x0 == 0);
+            const bool wr = (j == V - 1) && (x0 + V - 1 == n - 1);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if (wl) f[i] = wall_p0<T>(i, -1, y, z, n, a.lid_eq);
+                if (wr) f[i] = wall_p0<T>(i, +1, y, z, n, a.lid_eq);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) g[i][j] = codec.encode(f[i], w_of<T>(i));
+    }
+
+    const int64_t oc0 = int64_t(n) * r;
+    S* orow = dst + (19 * oc0 - 18 * (oc0 & smask));
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        SV o;
+#pragma unroll
+        for (int j = 0; j < V; ++j) o[j] = g[i][j];
+        __builtin_nontemporal_store(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+    }
+
+    if (a.update_macro) {
+        // rho and u of the lane's V cells, in chunks of at most 16 bytes
+        const int64_t c = int64_t(x0) + oc0;
+        constexpr int CH = int(16 / sizeof(T)) < V ? int(16 / sizeof(T)) : V;
+#pragma unroll
+        for (int j0 = 0; j0 < V; j0 += CH) {
+            if constexpr (CH == 1) {
+                a.rho[c + j0] = rho_v[j0];
+                a.u[c + j0] = ux_v[j0];
+                a.u[a.n3 + c + j0] = uy_v[j0];
+                a.u[2 * a.n3 + c + j0] = uz_v[j0];
+            } else {
+                typedef T TC __attribute__((ext_vector_type(CH)));
+                TC vr, vx, vy, vz;
+#pragma unroll
+                for (int k = 0; k < CH; ++k) {
+                    vr[k] = rho_v[j0 + k];
+                    vx[k] = ux_v[j0 + k];
+                    vy[k] = uy_v[j0 + k];
+                    vz[k] = uz_v[j0 + k];
+                }
+                *reinterpret_cast<TC*>(a.rho + c + j0) = vr;
+                *reinterpret_cast<TC*>(a.u + c + j0) = vx;
+                *reinterpret_cast<TC*>(a.u + a.n3 + c + j0) = vy;
+                *reinterpret_cast<TC*>(a.u + 2 * a.n3 + c + j0) = vz;
+            }
+        }
+    }
+    if (a.diagnostics) {
+        __shared__ double lds[2 * 16];
+        block_reduce2(m, e, lds);
+        if (threadIdx.x == 0) {
+            a.partials[2 * lb] = m;
+            a.partials[2 * lb + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Generic kernel: any N, any stride, any block size.  One thread per cell of a
 // 1-D grid over all N^3 cells, full 64-bit slot arithmetic per population.
 // ---------------------------------------------------------------------------
-template <class T, int KIND>
+template <class T, int KIND, bool FAST>
 __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a) {
     using C = Codec<T, KIND>;
     using S = typename C::S;
@@ -370,7 +671,7 @@ __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a
             const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
             f[i] = codec.load(src + slot(s, i, a.stride), w_of<T>(i));
         }
-        collide<T>(t, f, rho, ux, uy, uz, a.k);
+        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
 #pragma unroll
         for (int i = 0; i < Q; ++i) codec.store(dst + slot(c, i, a.stride), w_of<T>(i), f[i]);
         if (a.update_macro && is_store_macro(t)) {

@@ -56,6 +56,8 @@ class Params(ctypes.Structure):
         ("stride", ctypes.c_int64),
         ("viscosity", ctypes.c_double),
         ("velocity", ctypes.c_double),
+        ("fast", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -126,10 +128,11 @@ def kernel_literal(value: float, precision: str) -> float:
     return float(_libc.strtod(text.encode(), None))
 
 
-def params(dim, stride, viscosity, velocity, precision="f32", threads=None) -> Params:
+def params(dim, stride, viscosity, velocity, precision="f32", threads=None, fast=False) -> Params:
     if threads is None:
         threads = min(16, os.cpu_count() or 1)
-    return Params(dim, threads, stride, kernel_literal(viscosity, precision), kernel_literal(velocity, precision))
+    return Params(dim, threads, stride, kernel_literal(viscosity, precision), kernel_literal(velocity, precision),
+                  int(bool(fast)), 0)
 
 
 def constants(p: Params, precision="f32"):
@@ -183,12 +186,12 @@ def bf16_to_f32(a):
 class PushSim:
     """The reference form, driven like LBMCL<T> (lbmcl.hpp:435-521)."""
 
-    def __init__(self, dim, stride=None, viscosity=0.0089, velocity=0.05, precision="f32", threads=None):
+    def __init__(self, dim, stride=None, viscosity=0.0089, velocity=0.05, precision="f32", threads=None, fast=False):
         self.N = dim
         self.stride = stride if stride is not None else dim
         self.precision = precision
         self.dt = np.float32 if precision == "f32" else np.float64
-        self.p = params(dim, self.stride, viscosity, velocity, precision, threads)
+        self.p = params(dim, self.stride, viscosity, velocity, precision, threads, fast)
         n3 = dim ** 3
         self.f_stream = np.empty(n3 * Q, dtype=self.dt)
         self.f_collide = np.empty(n3 * Q, dtype=self.dt)
@@ -223,10 +226,12 @@ class PushSim:
 
 
 class PullSim:
-    """The gather form (native, FP16- or BF16-shifted storage)."""
+    """The gather form (native, FP16- or BF16-shifted storage).
+
+    fast selects the fused collision; the shifted storage forms always use it."""
 
     def __init__(self, dim, stride=None, viscosity=0.0089, velocity=0.05, precision="f32",
-                 storage=STORE_NATIVE, threads=None):
+                 storage=STORE_NATIVE, threads=None, fast=False):
         if storage != STORE_NATIVE and precision != "f32":
             raise ValueError("shifted storage computes in FP32")
         self.N = dim
@@ -235,7 +240,8 @@ class PullSim:
         self.storage = storage
         self.dt = np.float32 if precision == "f32" else np.float64
         self.st = self.dt if storage == STORE_NATIVE else np.uint16
-        self.p = params(dim, self.stride, viscosity, velocity, precision, threads)
+        self.fast = bool(fast) or storage != STORE_NATIVE
+        self.p = params(dim, self.stride, viscosity, velocity, precision, threads, self.fast)
         n3 = dim ** 3
         self.A = np.empty(n3 * Q, dtype=self.st)   # read by odd iterations
         self.B = np.empty(n3 * Q, dtype=self.st)   # read by even iterations

@@ -84,6 +84,8 @@ struct Params {
     int64_t stride;
     double viscosity;   // value of the VISCOSITY literal
     double velocity;    // value of the VELOCITY literal
+    int32_t fast;       // nonzero: the fused collision (no counterpart in the reference)
+    int32_t reserved;
 };
 
 // Slot of population q of cell id in the CSoA layout (kernels.cl:64).
@@ -97,12 +99,16 @@ template <class T> struct Consts {
     T w[Q];
     T inv_tau;
     T U;
+    T om1;       // 1 - inv_tau
+    bool fast;
     explicit Consts(const Params& p) {
         const T one = T(1), three = T(3), half = T(0.5);
         const T nu = T(p.viscosity);
         U = T(p.velocity);
         const T tau = (three * nu) + half;
         inv_tau = one / tau;
+        om1 = one - inv_tau;
+        fast = p.fast != 0;
         const T w0 = one / T(3), w1 = one / T(18), w7 = one / T(36);
         for (int i = 0; i < Q; ++i) w[i] = (i == 0) ? w0 : (i <= 6 ? w1 : w7);
     }
@@ -135,6 +141,52 @@ inline void moment_u(const T* f, T rho, T& ux, T& uy, T& uz) {
     ux = ux / rho;
     uy = uy / rho;
     uz = uz / rho;
+}
+
+// Fused collision (no counterpart in the reference; the HIP path uses it for
+// 16-bit storage and, on request, for native storage): the same moments and
+// boundary handling, and the BGK update written as
+//   f_i' = (1 - 1/tau) f_i + (rho w_i / tau) T_i,  T_i = K + e (3 + 4.5 e),
+//   K = 1 - 1.5 u2,  e = e_i . u,
+// with fused multiply-adds, T shared between opposite populations as
+// A +- 3e, A = 4.5 e e + K.  Each std::fma is a single IEEE rounding.
+template <class T>
+inline void collide_fused(const Consts<T>& k, int32_t t, T* f, T rho, T ux, T uy, T uz, const T* force) {
+    const bool moving = is_moving(t);
+    if (!moving && !is_fluid(t)) {
+        if (is_bounceback(t)) {
+            const int half[9] = {1, 2, 5, 7, 8, 11, 12, 13, 14};
+            for (int h : half) {
+                const T tmp = f[h];
+                f[h] = f[OPP[h]];
+                f[OPP[h]] = tmp;
+            }
+        }
+        return;
+    }
+    const T u2 = std::fma(uz, uz, std::fma(uy, uy, ux * ux));
+    const T K = std::fma(T(-1.5), u2, T(1));
+    const T s = moving ? rho : k.inv_tau * rho;
+    const T c0 = s * k.w[0], c1 = s * k.w[1], c7 = s * k.w[7];
+    const T keep = moving ? T(0) : k.om1;
+    const int PI[9] = {1, 2, 6, 7, 10, 15, 16, 11, 12};
+    const int PO[9] = {3, 4, 5, 9, 8, 13, 14, 17, 18};
+    f[0] = std::fma(c0, K, keep * f[0]);
+    for (int p = 0; p < 9; ++p) {
+        const int i = PI[p], o = PO[p];
+        const T e = ((ux * T(EX[i])) + (uy * T(EY[i]))) + (uz * T(EZ[i]));
+        const T A = std::fma(T(4.5) * e, e, K);
+        const T B = T(3) * e;
+        const T c = (i <= 6) ? c1 : c7;
+        f[i] = std::fma(c, A + B, keep * f[i]);
+        f[o] = std::fma(c, A - B, keep * f[o]);
+    }
+    if (force && is_fluid(t)) {
+        for (int i = 0; i < Q; ++i) {
+            const T eg = ((force[0] * T(EX[i])) + (force[1] * T(EY[i]))) + (force[2] * T(EZ[i]));
+            f[i] = f[i] + (T(3) * k.w[i]) * eg;
+        }
+    }
 }
 
 // The local part of the compute kernel (kernels.cl:343-418): lid fix-up,
@@ -171,6 +223,10 @@ inline void collide(const Consts<T>& k, int32_t t, T* f, T* rho_out, T* ux_out, 
         *ux_out = ux;
         *uy_out = uy;
         *uz_out = uz;
+    }
+    if (k.fast) {
+        collide_fused(k, t, f, rho, ux, uy, uz, force);
+        return;
     }
     const T u2 = ((ux * ux) + (uy * uy)) + (uz * uz);  // kernels.cl:390
     if (is_moving(t)) {  // kernels.cl:393-398
@@ -371,14 +427,14 @@ struct Codec {
         if (kind == STORE_NATIVE) return static_cast<const T*>(base)[s];
         const uint16_t h = static_cast<const uint16_t*>(base)[s];
         const float v = (kind == STORE_FP16) ? f16_to_f32(h) : bf16_to_f32(h);
-        return T(float(w[i]) + v * inv_scale);
+        return T(std::fma(v, inv_scale, float(w[i])));  // = w + v / scale: the product is exact
     }
     inline void store(void* base, int64_t s, int i, T f) const {
         if (kind == STORE_NATIVE) {
             static_cast<T*>(base)[s] = f;
             return;
         }
-        const float v = (float(f) - float(w[i])) * scale;
+        const float v = std::fma(float(f), scale, -(float(w[i]) * scale));  // = (f - w) * scale, one rounding
         static_cast<uint16_t*>(base)[s] = (kind == STORE_FP16) ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
     }
 };

@@ -25,11 +25,11 @@ STORE = {"native": O.STORE_NATIVE, "fp16": O.STORE_FP16, "bf16": O.STORE_BF16}
 
 
 def _pair(n, stride, precision, storage="native", kernel=capi.LBM_KERNEL_AUTO, lws=(0, 0, 0),
-          viscosity=0.0089, velocity=0.05):
+          viscosity=0.0089, velocity=0.05, fast=False):
     g = capi.Context(n, viscosity=viscosity, velocity=velocity, stride=stride, precision=precision,
-                     storage=storage, kernel=kernel, lws=lws)
+                     storage=storage, kernel=kernel, lws=lws, optimize=fast)
     o = O.PullSim(n, stride=stride, viscosity=viscosity, velocity=velocity, precision=precision,
-                  storage=STORE[storage])
+                  storage=STORE[storage], fast=fast)
     g.initialize()
     o.initialize()
     return g, o
@@ -60,6 +60,15 @@ CASES = [
     (128, 64, "f32", capi.LBM_KERNEL_AUTO, 8),
     (128, 64, "f64", capi.LBM_KERNEL_AUTO, 8),
     (256, 256 ** 3, "f32", capi.LBM_KERNEL_ROW, 4),
+    (256, 4, "f32", capi.LBM_KERNEL_VEC, 5),
+    (256, 256 ** 3, "f32", capi.LBM_KERNEL_VEC, 5),
+    (128, 2, "f64", capi.LBM_KERNEL_VEC, 6),
+    (128, 128, "f64", capi.LBM_KERNEL_VEC, 6),
+    (512, 1024, "f64", capi.LBM_KERNEL_VEC, 2),
+    (256, 2, "f32", capi.LBM_KERNEL_VEC8, 5),
+    (128, 128, "f32", capi.LBM_KERNEL_VEC8, 6),
+    (64, 64 ** 3, "f64", capi.LBM_KERNEL_VEC8, 8),
+    (64, 1, "f64", capi.LBM_KERNEL_VEC8, 8),
 ]
 
 
@@ -76,31 +85,59 @@ def test_native_matches_oracle(n, stride, precision, kernel, steps):
             o.step(update_macro=True)
             if it in (1, 2, steps):
                 _assert_macro_equal(g, o, n, precision, f"{label} it={it}")
-        # full lattice, both buffers, in the reference's push view
+        # both lattices as stored, and (up to 256^3) in the reference's push view
         for buf, ob in ((0, o.A), (1, o.B)):
-            fg, fo = g.read_f(buf), o.view(ob)
-            assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
+            if n <= 256:
+                fg, fo = g.read_f(buf), o.view(ob)
+                assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
             sg = g.read_state(buf)
             assert nan_equal(sg, ob), f"{label}: stored state {buf} differs"
 
 
+FAST_CASES = [
+    (16, 16, "f32", capi.LBM_KERNEL_GENERIC, 10),
+    (64, 64, "f64", capi.LBM_KERNEL_ROW, 10),
+    (128, 64, "f32", capi.LBM_KERNEL_VEC8, 8),
+    (256, 256, "f32", capi.LBM_KERNEL_VEC, 5),
+    (128, 128, "f64", capi.LBM_KERNEL_VEC, 6),
+]
+
+
+@pytest.mark.parametrize("n,stride,precision,kernel,steps", FAST_CASES)
+def test_fused_collision_matches_oracle(n, stride, precision, kernel, steps):
+    """-o: the fused collision, bit for bit against the oracle's fused form."""
+    label = f"fast n={n} s={stride} {precision} k={kernel}"
+    g, o = _pair(n, stride, precision, kernel=kernel, fast=True)
+    with g:
+        for it in range(1, steps + 1):
+            g.compute(it, update_macro=True)
+            o.step(update_macro=True)
+        _assert_macro_equal(g, o, n, precision, label)
+        for buf, ob in ((0, o.A), (1, o.B)):
+            assert nan_equal(g.read_state(buf), ob), f"{label}: stored state {buf} differs"
+
+
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-@pytest.mark.parametrize("n,kernel", [(16, capi.LBM_KERNEL_GENERIC), (64, capi.LBM_KERNEL_ROW)])
-def test_shifted_storage_matches_oracle(storage, n, kernel):
+@pytest.mark.parametrize("n,kernel,steps", [(16, capi.LBM_KERNEL_GENERIC, 12), (64, capi.LBM_KERNEL_ROW, 12),
+                                            (256, capi.LBM_KERNEL_VEC8, 4), (512, capi.LBM_KERNEL_VEC, 2)])
+def test_shifted_storage_matches_oracle(storage, n, kernel, steps):
     label = f"{storage} n={n} k={kernel}"
     g, o = _pair(n, n, "f32", storage=storage, kernel=kernel)
     with g:
-        for it in range(1, 13):
+        for it in range(1, steps + 1):
             g.compute(it, update_macro=True)
             o.step(update_macro=True)
         _assert_macro_equal(g, o, n, "f32", label)
         for buf, ob in ((0, o.A), (1, o.B)):
+            if n > 256:
+                continue
             fg, fo = g.read_f(buf), o.view(ob)
             assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
 
 
 @pytest.mark.parametrize("n,precision,kernel", [(16, "f32", capi.LBM_KERNEL_GENERIC), (64, "f64", capi.LBM_KERNEL_ROW),
-                                                (128, "f32", capi.LBM_KERNEL_ROW)])
+                                                (128, "f32", capi.LBM_KERNEL_ROW), (256, "f32", capi.LBM_KERNEL_VEC),
+                                                (128, "f64", capi.LBM_KERNEL_VEC)])
 def test_diagnostics(n, precision, kernel):
     g, o = _pair(n, n, precision, kernel=kernel)
     with g:
