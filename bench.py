@@ -2,8 +2,9 @@
 """Throughput of the D3Q19 lid-driven-cavity collide-stream step on one MI355X.
 
 Workload: the 512^3 cavity (nu = 0.0089, lid u = 0.05), FP32 populations in
-the CSoA layout with stride 512^3 (planar populations), the row kernel.  A
-"step" is one collide-stream iteration of the whole lattice.  The timed region
+the CSoA layout with stride 64, the vector row kernel (8-byte lanes), exact
+(uncontracted) arithmetic.  A "step" is one collide-stream iteration of the
+whole lattice.  The timed region
 holds exactly --steps iterations enqueued back to back on the library's stream,
 bracketed by device synchronisation; the lattice is resident in HBM and already
 initialised before it starts.
@@ -37,11 +38,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True):
+def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True,
+                fused=False):
     from lbmcl_amd import capi
     stride = stride or dim ** 3
     with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=True,
-                      xcd_remap=xcd_remap) as g:
+                      xcd_remap=xcd_remap, optimize=fused) as g:
         g.initialize()
         it = 1
         if warmup:
@@ -88,11 +90,11 @@ def copy_bandwidth(nbytes=4 << 30, reps=5):
     return {"bytes": nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
 
 
-def cpu_baseline(dim, steps, threads):
+def cpu_baseline(dim, steps, threads, stride):
     """The CPU oracle (reference push form, OpenMP) on `steps` iterations of the same lattice."""
     import oracle as O
     O.build()
-    sim = O.PushSim(dim, stride=dim ** 3, precision="f32", threads=threads)
+    sim = O.PushSim(dim, stride=stride, precision="f32", threads=threads)
     sim.initialize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -100,14 +102,14 @@ def cpu_baseline(dim, steps, threads):
     dt = time.perf_counter() - t0
     return {
         "value": (dim - 2) ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-        "sample": f"oracle push form, {dim}^3 FP32 stride {dim}^3, {steps} iterations, {threads} OpenMP threads, "
+        "sample": f"oracle push form, {dim}^3 FP32 stride {stride}, {steps} iterations, {threads} OpenMP threads, "
                   f"{dt:.1f} s",
     }
 
 
 def traffic_from_profiles(dim, precision, storage):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, if committed."""
-    path = os.path.join(ROOT, "profiles", f"collide_stream_row_{dim}_{precision}_{storage}_pmc.md")
+    path = os.path.join(ROOT, "profiles", f"collide_stream_vec_{dim}_{precision}_{storage}_pmc.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -121,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dim", type=int, default=512)
-    ap.add_argument("--stride", type=int, default=0, help="CSoA stride (default dim^3)")
+    ap.add_argument("--stride", type=int, default=64, help="CSoA stride")
     ap.add_argument("--variant-steps", type=int, default=300, help="steps of the FP16/BF16/FP64 runs (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="iterations of the CPU oracle sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -135,7 +137,7 @@ def main():
         return 2
     rank = int(os.environ.get("RANK", "0"))
     dim = args.dim
-    stride = args.stride or dim ** 3
+    stride = args.stride
 
     log(f"[bench] main run: {dim}^3 f32 native stride {stride}, warmup {args.warmup}, steps {args.steps}")
     main_run = run_lattice(dim, args.steps, args.warmup, "f32", "native", stride, args.kernel, not args.no_remap)
@@ -144,13 +146,15 @@ def main():
 
     variants = {}
     if args.variant_steps > 0:
-        for prec, st in (("f32", "fp16"), ("f32", "bf16"), ("f64", "native")):
-            log(f"[bench] variant {prec} {st}")
+        for prec, st, fused in (("f32", "fp16", True), ("f32", "bf16", True), ("f64", "native", False),
+                                ("f32", "native", True)):
+            name = f"{prec}_{st}" + ("_fused" if st == "native" and fused else "")
+            log(f"[bench] variant {name}")
             r = run_lattice(dim, args.variant_steps, max(5, args.warmup // 4), prec, st, stride, args.kernel,
-                            not args.no_remap)
-            variants[f"{prec}_{st}"] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs",
-                                                         "frac", "bytes_per_update", "steps")}
-            log(f"[bench] {prec} {st}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
+                            not args.no_remap, fused)
+            variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
+                                                "bytes_per_update", "steps")}
+            log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
         variants["fp16_speedup_vs_f32"] = variants["f32_fp16"]["mlups"] / main_run["mlups"]
 
     log("[bench] copy bandwidth")
@@ -158,7 +162,7 @@ def main():
     cpu = None
     if args.cpu_steps > 0 and rank == 0:
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle iterations at {dim}^3")
-        cpu = cpu_baseline(dim, args.cpu_steps, args.cpu_threads)
+        cpu = cpu_baseline(dim, args.cpu_steps, args.cpu_threads, stride)
 
     wet = (dim - 2) ** 3
     out = {
@@ -175,12 +179,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (deterministic lid-driven cavity initial state; no dataset)",
         "config": {"workload": f"D3Q19 BGK lid-driven cavity {dim}^3, nu=0.0089, lid u=0.05, FP32 populations, "
-                               f"CSoA stride {stride}, pull-form row kernel",
+                               f"CSoA stride {stride}, pull-form vector row kernel, exact arithmetic",
                    "dim": dim, "stride": stride, "wet_cells": wet, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_run["frac"], 4),
                      "traffic": traffic_from_profiles(dim, "f32", "native"),
-                     "kernel": "lbm_collide_stream_row<float, native>",
+                     "kernel": "lbm_collide_stream_vec<float, native, 8-byte lanes, exact>",
                      "avg_kernel_ms": round(main_run["avg_kernel_ms"], 4),
                      "bytes_per_launch": wet * 152},
         "cpu_baseline": cpu,

@@ -1,8 +1,10 @@
 // lbm_io_selftest.cpp -- writes the three dump formats from deterministic
-// synthetic arrays, so the formats can be checked without a GPU.
+// synthetic arrays, plus the arrays themselves (rho.bin, u.bin, f.bin,
+// map.bin), so the formats can be checked without a GPU.
 //
 // usage: lbm_io_selftest <outdir> <dim> <stride> <f32|f64> <iteration> <iterations>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -25,6 +27,17 @@ int run(const std::string& dir, size_t dim, size_t stride, size_t it, size_t its
     }
     for (size_t s = 0; s < 19 * n3; ++s)
         f[s] = (s % 13 == 5) ? std::numeric_limits<T>::quiet_NaN() : T(1) / T(36) + T(1e-4) * std::cos(T(s));
+    auto raw = [&](const char* name, const void* p, size_t bytes) {
+        std::FILE* fh = std::fopen((dir + "/" + name).c_str(), "wb");
+        if (fh) {
+            std::fwrite(p, 1, bytes, fh);
+            std::fclose(fh);
+        }
+    };
+    raw("rho.bin", rho.data(), rho.size() * sizeof(T));
+    raw("u.bin", u.data(), u.size() * sizeof(T));
+    raw("f.bin", f.data(), f.size() * sizeof(T));
+    raw("map.bin", map.data(), map.size() * sizeof(int32_t));
     lbm_io::write_map(dir + "/map.dump", map.data(), dim);
     lbm_io::write_f(lbm_io::numbered_path(dir, "f_", it, its, ".dump"), f.data(), dim, stride);
     lbm_io::write_vti(lbm_io::numbered_path(dir, "lbmcl.", it, its, ".vti"), rho.data(), u.data(), dim);
