@@ -86,8 +86,32 @@ typedef struct lbm_config {
     int32_t optimize;    /* -o: fused collision (see lbm_compute); 16-bit storage always uses it */
     int32_t kernel;      /* LBM_KERNEL_* */
     int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
-    int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernel) */
+    int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernels) */
+    int32_t raw_parameters; /* nonzero: use viscosity and velocity as given, without the 6-digit
+                               rounding of the reference's -D literals (for gradient checks) */
 } lbm_config;
+
+/* objectives of lbm_adjoint (no counterpart in the reference) */
+#define LBM_OBJECTIVE_KINETIC_ENERGY 0  /* J = sum over FLUID/MOVING cells of rho |u|^2 / 2 */
+#define LBM_OBJECTIVE_MEAN_UX_PLANE 1   /* J = mean u_x over the FLUID/MOVING cells of plane z */
+
+typedef struct lbm_adjoint_config {
+    int32_t objective;         /* LBM_OBJECTIVE_* */
+    int32_t plane_z;           /* plane of LBM_OBJECTIVE_MEAN_UX_PLANE, 1 .. dim-2 */
+    int64_t steps;             /* T: the objective is evaluated on the state after T iterations */
+    int64_t checkpoint_every;  /* K: forward states kept every K steps; 0 = round(sqrt(T)) */
+} lbm_adjoint_config;
+
+typedef struct lbm_adjoint_result {
+    double objective;     /* J */
+    double d_velocity;    /* dJ/dU (lid velocity) */
+    double d_viscosity;   /* dJ/dnu */
+    double forward_ms;    /* forward pass with checkpointing */
+    double adjoint_ms;    /* objective seed, recomputation and reverse sweep */
+    double device_bytes;  /* device memory held during the run */
+    int64_t checkpoints;  /* number of checkpoints (segments) */
+    int64_t segment;      /* K */
+} lbm_adjoint_result;
 
 typedef struct lbm_ctx lbm_ctx;
 
@@ -174,6 +198,23 @@ int lbm_total_time_ms(lbm_ctx* ctx, double* ms);
 int lbm_kernels_time_ms(lbm_ctx* ctx, double* ms);
 /* Forget recorded events (keeps the context). */
 int lbm_clear_events(lbm_ctx* ctx);
+
+/* Body force g(x) on FLUID cells: 3*dim^3 values [gx | gy | gz] of the
+ * arithmetic type, copied to the device; each collision then adds
+ * 3 w_i (e_i . g) to the populations (a zero field changes nothing).  NULL
+ * removes the force.  No counterpart in the reference. */
+int lbm_set_force(lbm_ctx* ctx, const void* force);
+
+/* Run the cavity from its initial state for config->steps iterations and
+ * return the objective J and its gradients with respect to the lid velocity,
+ * the viscosity and (if grad_force is not NULL: 3*dim^3 doubles) the body
+ * force, by the adjoint collide-stream step with checkpoints every
+ * config->checkpoint_every steps and recomputation of the segments.  Native
+ * storage only; the arithmetic type of the context is used for the adjoint
+ * states, FP64 for the gradient sums.  The context's lattices are left
+ * unspecified (call lbm_initialize before computing again).  No counterpart
+ * in the reference. */
+int lbm_adjoint(lbm_ctx* ctx, const lbm_adjoint_config* config, lbm_adjoint_result* result, double* grad_force);
 
 /* Text of the calling thread's last error ("" if none). */
 const char* lbm_last_error(void);

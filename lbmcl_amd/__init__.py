@@ -7,9 +7,8 @@ Native parts (built in-tree by ``python -m lbmcl_amd.build``):
 
 Python parts:
   capi       ctypes binding of the C ABI (no CPU fallback)
-  simulation ``LBMCL``, the reference simulation class (lbmcl.hpp:33-679) over the C ABI
   vti        numpy-only VTI reader
   verify     the verify.py counterpart (MSE / max-abs error per dumped iteration)
-  adjoint    gradients of cavity objectives (torch.autograd.Function)
+  accuracy   the storage/precision forms against an FP64 run
 """
-__all__ = ["capi", "vti", "verify", "simulation"]
+__all__ = ["capi", "vti", "verify", "accuracy", "build"]

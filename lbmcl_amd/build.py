@@ -45,7 +45,7 @@ def _run(cmd):
 
 def build_lib():
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
-          os.path.join(CSRC, "lbm_capi.hip"), "-o", LIB])
+          os.path.join(CSRC, "lbm_capi.hip"), os.path.join(CSRC, "lbm_adjoint.hip"), "-o", LIB])
 
 
 def build_membw():

@@ -35,8 +35,10 @@ EXPORTS = (
     "lbm_compute", "lbm_compute_range", "lbm_finish", "lbm_read_map", "lbm_read_rho", "lbm_read_u",
     "lbm_read_f", "lbm_read_state", "lbm_write_state", "lbm_read_diagnostics", "lbm_get_constants",
     "lbm_event_count", "lbm_event_get", "lbm_total_time_ms", "lbm_kernels_time_ms", "lbm_clear_events",
-    "lbm_last_error",
+    "lbm_set_force", "lbm_adjoint", "lbm_last_error",
 )
+
+LBM_OBJECTIVE_KINETIC_ENERGY, LBM_OBJECTIVE_MEAN_UX_PLANE = 0, 1
 
 
 class LbmConfig(ctypes.Structure):
@@ -53,6 +55,29 @@ class LbmConfig(ctypes.Structure):
         ("kernel", ctypes.c_int32),
         ("profile", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32),
+        ("raw_parameters", ctypes.c_int32),
+    ]
+
+
+class LbmAdjointConfig(ctypes.Structure):
+    _fields_ = [
+        ("objective", ctypes.c_int32),
+        ("plane_z", ctypes.c_int32),
+        ("steps", ctypes.c_int64),
+        ("checkpoint_every", ctypes.c_int64),
+    ]
+
+
+class LbmAdjointResult(ctypes.Structure):
+    _fields_ = [
+        ("objective", ctypes.c_double),
+        ("d_velocity", ctypes.c_double),
+        ("d_viscosity", ctypes.c_double),
+        ("forward_ms", ctypes.c_double),
+        ("adjoint_ms", ctypes.c_double),
+        ("device_bytes", ctypes.c_double),
+        ("checkpoints", ctypes.c_int64),
+        ("segment", ctypes.c_int64),
     ]
 
 
@@ -103,6 +128,8 @@ def _declare(L):
         "lbm_total_time_ms": [vp, P(dbl)],
         "lbm_kernels_time_ms": [vp, P(dbl)],
         "lbm_clear_events": [vp],
+        "lbm_set_force": [vp, vp],
+        "lbm_adjoint": [vp, P(LbmAdjointConfig), P(LbmAdjointResult), vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -141,7 +168,8 @@ class Context:
     """One device lattice: a thin owner of an lbm_ctx."""
 
     def __init__(self, dim, viscosity=0.0089, velocity=0.05, stride=None, precision="f32", storage="native",
-                 device=0, lws=(0, 0, 0), kernel=LBM_KERNEL_AUTO, profile=True, xcd_remap=True, optimize=False):
+                 device=0, lws=(0, 0, 0), kernel=LBM_KERNEL_AUTO, profile=True, xcd_remap=True, optimize=False,
+                 raw_parameters=False):
         self.dim = int(dim)
         self.precision = precision
         self.storage = storage
@@ -163,6 +191,7 @@ class Context:
         cfg.kernel = kernel
         cfg.profile = int(bool(profile))
         cfg.xcd_remap = int(bool(xcd_remap))
+        cfg.raw_parameters = int(bool(raw_parameters))
         self.cfg = cfg
         self.stride = cfg.stride
         h = ctypes.c_void_p()
@@ -245,6 +274,28 @@ class Context:
         out = (ctypes.c_double * 4)()
         check(lib().lbm_get_constants(self._h, out), "lbm_get_constants")
         return {"inv_tau": out[0], "velocity": out[1], "viscosity": out[2], "scale": out[3]}
+
+    # -- body force and adjoint -----------------------------------------------
+    def set_force(self, force):
+        """Body force g as an array of 3*dim^3 values [gx | gy | gz], or None."""
+        if force is None:
+            check(lib().lbm_set_force(self._h, None), "lbm_set_force")
+            return
+        g = np.ascontiguousarray(force, dtype=self.dtype).reshape(-1)
+        assert g.size == 3 * self.n3
+        check(lib().lbm_set_force(self._h, _ptr(g)), "lbm_set_force")
+
+    def adjoint(self, steps, objective=LBM_OBJECTIVE_KINETIC_ENERGY, plane_z=0, checkpoint_every=0,
+                grad_force=True):
+        """J and dJ/dU, dJ/dnu (and dJ/dg, shape (3, dim^3), if grad_force) after `steps` iterations."""
+        cfg = LbmAdjointConfig(objective, plane_z, steps, checkpoint_every)
+        res = LbmAdjointResult()
+        g = np.zeros(3 * self.n3, dtype=np.float64) if grad_force else None
+        check(lib().lbm_adjoint(self._h, ctypes.byref(cfg), ctypes.byref(res), None if g is None else _ptr(g)),
+              "lbm_adjoint")
+        out = {f: getattr(res, f) for f, _ in LbmAdjointResult._fields_}
+        out["d_force"] = None if g is None else g.reshape(3, self.n3)
+        return out
 
     # -- timing ---------------------------------------------------------------
     def events(self):

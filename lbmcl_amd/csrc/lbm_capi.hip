@@ -4,347 +4,9 @@
 // schedule of the compute kernel, blocking reads and event timing.  It takes
 // the place of the OpenCL device layer of LBMCL (lbmcl.hpp:392-593,
 // libs/CLUtil.hpp).
-#include <hip/hip_runtime.h>
+#include "lbm_ctx.hpp"
 
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "../../include/lbm_hip.h"
-#include "lbm_kernels.hpp"
-
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& what) {
-    g_last_error = what;
-    return code;
-}
-
-#define LBM_HIP_CHECK(call, what)                                                                 \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(LBM_ERR_RUNTIME, std::string(what) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
-bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
-
-int ilog2(int64_t v) {
-    int k = 0;
-    while ((int64_t(1) << k) < v) ++k;
-    return k;
-}
-
-// The reference passes nu and U to the OpenCL compiler as -D literals printed
-// by an ostream with 6 significant digits (lbmcl.hpp:140-141); in FP32 builds
-// the literal is single precision (lbmcl.hpp:146).  Reproduce that rounding.
-double kernel_literal(double value, int precision) {
-    char buf[64];
-    if (precision == LBM_F32) {
-        std::snprintf(buf, sizeof buf, "%g", double(float(value)));
-        return double(std::strtof(buf, nullptr));
-    }
-    std::snprintf(buf, sizeof buf, "%g", value);
-    return std::strtod(buf, nullptr);
-}
-
-// Power-of-two scale of the shifted storage: 2^floor(log2(1024 / (0.05 + 2|U|))),
-// clamped to [2^0, 2^24].  |f_i - w_i| stays far below 1 in the cavity, so
-// scaled values stay in binary16's normal range with ample headroom.
-double shift_scale(double U) {
-    const double a = 0.05 + 2.0 * std::fabs(U);
-    int k = int(std::floor(std::log2(1024.0 / a)));
-    if (k < 0) k = 0;
-    if (k > 24) k = 24;
-    return std::ldexp(1.0, k);
-}
-
-struct EventRec {
-    int32_t kind;
-    hipEvent_t start;
-    hipEvent_t stop;
-};
-
-}  // namespace
-
-struct lbm_ctx {
-    lbm_config cfg;
-    int32_t n = 0;
-    int32_t log2n = 0;
-    int64_t n3 = 0;
-    int64_t stride = 0;
-    size_t tsize = 0;   // arithmetic type size
-    size_t ssize = 0;   // storage element size
-    hipStream_t stream = nullptr;
-    void* lat[2] = {nullptr, nullptr};  // lat[0] read by odd iterations, lat[1] by even
-    void* rho = nullptr;
-    void* u = nullptr;
-    int32_t* map = nullptr;
-    double* partials = nullptr;
-    int64_t partials_cap = 0;  // in blocks
-    double* diag = nullptr;    // 2 doubles
-    bool diag_valid = false;
-    bool initialized = false;
-    double nu = 0, inv_tau = 0, U = 0, scale = 1;
-    std::vector<EventRec> events;
-    std::vector<hipEvent_t> pool;
-};
-
-namespace {
-
-int set_device(lbm_ctx* c) {
-    LBM_HIP_CHECK(hipSetDevice(c->cfg.device), "hipSetDevice");
-    return LBM_OK;
-}
-
-int ev_begin(lbm_ctx* c, int kind, EventRec& r) {
-    r.kind = kind;
-    r.start = r.stop = nullptr;
-    if (!c->cfg.profile) return LBM_OK;
-    LBM_HIP_CHECK(hipEventCreate(&r.start), "hipEventCreate");
-    LBM_HIP_CHECK(hipEventCreate(&r.stop), "hipEventCreate");
-    LBM_HIP_CHECK(hipEventRecord(r.start, c->stream), "hipEventRecord");
-    return LBM_OK;
-}
-
-int ev_end(lbm_ctx* c, EventRec& r) {
-    if (!c->cfg.profile) return LBM_OK;
-    LBM_HIP_CHECK(hipEventRecord(r.stop, c->stream), "hipEventRecord");
-    c->events.push_back(r);
-    return LBM_OK;
-}
-
-void free_events(lbm_ctx* c) {
-    for (auto& e : c->events) {
-        if (e.start) (void)hipEventDestroy(e.start);
-        if (e.stop) (void)hipEventDestroy(e.stop);
-    }
-    c->events.clear();
-}
-
-void release(lbm_ctx* c) {
-    if (!c) return;
-    (void)hipSetDevice(c->cfg.device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    free_events(c);
-    for (void* p : {c->lat[0], c->lat[1], c->rho, c->u, (void*)c->map, (void*)c->partials, (void*)c->diag})
-        if (p) (void)hipFree(p);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
-}
-
-int dev_alloc(void** p, size_t bytes, const char* what) {
-    *p = nullptr;
-    const hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return fail(LBM_ERR_MEMORY, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
-                                        " bytes): " + hipGetErrorString(e));
-    }
-    return LBM_OK;
-}
-
-enum KernelKind { K_GENERIC, K_ROW, K_VEC, K_VEC8 };
-
-// bytes per lane and population of the vector kernels
-int64_t lane_bytes(KernelKind k) { return k == K_VEC8 ? 8 : 16; }
-
-bool vec_possible(const lbm_ctx* c, KernelKind k) {
-    const int64_t v = lane_bytes(k) / int64_t(c->ssize);  // cells per lane
-    return v >= 1 && c->stride >= v && c->n / v >= 64;
-}
-
-// Preferred vector width per storage type (measured on MI355X at 512^3):
-// 16-byte lanes for FP64 (2 cells per lane), 8-byte lanes for FP32 and 16-bit
-// storage, whose 16-byte forms need too many registers for the occupancy the
-// kernel wants.
-KernelKind preferred_vec(const lbm_ctx* c) { return c->ssize == 8 ? K_VEC : K_VEC8; }
-
-KernelKind kernel_kind(const lbm_ctx* c) {
-    switch (c->cfg.kernel) {
-        case LBM_KERNEL_GENERIC: return K_GENERIC;
-        case LBM_KERNEL_ROW: return K_ROW;
-        case LBM_KERNEL_VEC: return K_VEC;
-        case LBM_KERNEL_VEC8: return K_VEC8;
-        default: {
-            const KernelKind p = preferred_vec(c);
-            if (vec_possible(c, p)) return p;
-            if (vec_possible(c, K_VEC8)) return K_VEC8;
-            return c->n >= 64 ? K_ROW : K_GENERIC;
-        }
-    }
-}
-
-// cells per lane of the selected kernel
-int64_t vec_width(const lbm_ctx* c) {
-    const KernelKind k = kernel_kind(c);
-    return (k == K_VEC || k == K_VEC8) ? lane_bytes(k) / int64_t(c->ssize) : 1;
-}
-
-// threads per block and blocks per row of the row kernels
-int64_t row_block(const lbm_ctx* c) {
-    const int64_t per_row = c->n / vec_width(c);
-    return per_row < 256 ? per_row : 256;
-}
-int64_t row_gx(const lbm_ctx* c) { return (c->n / vec_width(c)) / row_block(c); }
-
-int generic_block(const lbm_ctx* c) {
-    int64_t b = 1;
-    for (int d = 0; d < 3; ++d) b *= (c->cfg.lws[d] > 0 ? c->cfg.lws[d] : 1);
-    if (c->cfg.lws[0] <= 0 && c->cfg.lws[1] <= 0 && c->cfg.lws[2] <= 0) b = 256;
-    return int(b);
-}
-
-// Number of blocks of one compute launch (for the diagnostics partials).
-int64_t compute_blocks(const lbm_ctx* c) {
-    const KernelKind k = kernel_kind(c);
-    if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n);  // all rows
-    if (k == K_ROW) return row_gx(c) * int64_t(c->n - 2) * int64_t(c->n - 2);         // interior rows
-    const int64_t b = generic_block(c);
-    return (c->n3 + b - 1) / b;
-}
-
-// The fused collision is used for 16-bit storage always and for native
-// storage when fast math (-o) is requested.
-bool use_fast(const lbm_ctx* c) { return c->cfg.storage != LBM_STORE_NATIVE || c->cfg.optimize != 0; }
-
-template <class T, int KIND, bool FAST>
-int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
-    lbm::StepArgs<T> a{};
-    a.src = src;
-    a.dst = dst;
-    a.rho = static_cast<T*>(c->rho);
-    a.u = static_cast<T*>(c->u);
-    a.partials = c->partials;
-    a.n3 = c->n3;
-    a.stride = c->stride;
-    a.n = c->n;
-    a.log2n = c->log2n;
-    a.update_macro = update_macro;
-    a.diagnostics = diag;
-    a.xcd_remap = c->cfg.xcd_remap;
-    a.k.inv_tau = T(c->inv_tau);
-    a.k.U = T(c->U);
-    a.k.om1 = T(1) - T(c->inv_tau);
-    a.scale = float(c->scale);
-    a.inv_scale = float(1.0 / c->scale);
-    {
-        // lid-layer initial populations, same expression as the initialize kernel
-        const T U = T(c->U);
-        const T u2 = ((U * U) + (T(0) * T(0))) + (T(0) * T(0));
-        for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq<T>(i, T(1), U, T(0), T(0), u2);
-    }
-    const KernelKind kk = kernel_kind(c);
-    if (kk != K_GENERIC) {
-        a.gx = int32_t(row_gx(c));
-        a.log2gx = ilog2(a.gx);
-        const int64_t nb = compute_blocks(c);
-        const unsigned bx = unsigned(row_block(c));
-        if (kk == K_VEC)
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 16, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
-                               a);
-        else if (kk == K_VEC8)
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 8, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
-                               a);
-        else
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_row<T, KIND, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream, a);
-    } else {
-        const int b = generic_block(c);
-        const int64_t nb = (c->n3 + b - 1) / b;
-        a.gx = 0;
-        hipLaunchKernelGGL((lbm::lbm_collide_stream_generic<T, KIND, FAST>), dim3(unsigned(nb)), dim3(b), 0, c->stream,
-                           a);
-    }
-    LBM_HIP_CHECK(hipGetLastError(), "compute launch");
-    return LBM_OK;
-}
-
-template <class T, int KIND>
-int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
-    if constexpr (KIND != lbm::ST_NATIVE) {
-        return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
-    } else {
-        if (use_fast(c)) return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
-        return launch_compute_impl<T, KIND, false>(c, src, dst, update_macro, diag);
-    }
-}
-
-template <class T, int KIND>
-int launch_initialize(lbm_ctx* c) {
-    lbm::InitArgs<T> a{};
-    a.a = c->lat[0];
-    a.b = c->lat[1];
-    a.rho = static_cast<T*>(c->rho);
-    a.u = static_cast<T*>(c->u);
-    a.map = c->map;
-    a.n3 = c->n3;
-    a.stride = c->stride;
-    a.n = c->n;
-    a.log2n = c->log2n;
-    a.U = T(c->U);
-    a.scale = float(c->scale);
-    a.inv_scale = float(1.0 / c->scale);
-    const int64_t nb = (c->n3 + 255) / 256;
-    hipLaunchKernelGGL((lbm::lbm_initialize_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
-    LBM_HIP_CHECK(hipGetLastError(), "initialize launch");
-    return LBM_OK;
-}
-
-template <class T, int KIND>
-int launch_view(lbm_ctx* c, const void* p, T* out) {
-    lbm::ViewArgs<T> a{};
-    a.p = p;
-    a.out = out;
-    a.n3 = c->n3;
-    a.stride = c->stride;
-    a.n = c->n;
-    a.log2n = c->log2n;
-    a.scale = float(c->scale);
-    a.inv_scale = float(1.0 / c->scale);
-    const int64_t nb = (c->n3 + 255) / 256;
-    hipLaunchKernelGGL((lbm::lbm_view_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
-    LBM_HIP_CHECK(hipGetLastError(), "view launch");
-    return LBM_OK;
-}
-
-// Dispatch on (precision, storage).
-#define LBM_DISPATCH(c, FN, ...)                                                        \
-    ((c)->cfg.precision == LBM_F64 ? FN<double, lbm::ST_NATIVE>(__VA_ARGS__)            \
-     : (c)->cfg.storage == LBM_STORE_FP16_SHIFT ? FN<float, lbm::ST_FP16>(__VA_ARGS__)  \
-     : (c)->cfg.storage == LBM_STORE_BF16_SHIFT ? FN<float, lbm::ST_BF16>(__VA_ARGS__)  \
-                                                : FN<float, lbm::ST_NATIVE>(__VA_ARGS__))
-
-int read_device(lbm_ctx* c, int kind, void* dst, const void* src, size_t bytes) {
-    EventRec r;
-    int rc = ev_begin(c, kind, r);
-    if (rc) return rc;
-    LBM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync D2H");
-    rc = ev_end(c, r);
-    if (rc) return rc;
-    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    return LBM_OK;
-}
-
-#define LBM_CHECK_CTX(c)                                                   \
-    do {                                                                   \
-        if (!(c)) return fail(LBM_ERR_INVALID, "null context");            \
-        int rc_ = set_device(c);                                           \
-        if (rc_) return rc_;                                               \
-    } while (0)
-
-#define LBM_CHECK_INIT(c)                                                                \
-    do {                                                                                 \
-        if (!(c)->initialized) return fail(LBM_ERR_STATE, "lbm_initialize not called");  \
-    } while (0)
-
-}  // namespace
+using namespace lbm_host;
 
 extern "C" {
 
@@ -429,8 +91,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     c->stride = k.stride;
     c->tsize = (k.precision == LBM_F64) ? 8 : 4;
     c->ssize = (k.storage == LBM_STORE_NATIVE) ? c->tsize : 2;
-    c->nu = kernel_literal(k.viscosity, k.precision);
-    c->U = kernel_literal(k.velocity, k.precision);
+    c->nu = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.viscosity)) : k.viscosity)
+                             : kernel_literal(k.viscosity, k.precision);
+    c->U = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.velocity)) : k.velocity)
+                            : kernel_literal(k.velocity, k.precision);
     if (k.precision == LBM_F32) {
         const float nu = float(c->nu);
         const float tau = (3.0f * nu) + 0.5f;  // kernels.cl:61, folded in float

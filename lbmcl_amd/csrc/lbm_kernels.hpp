@@ -269,6 +269,16 @@ __device__ __forceinline__ void collide_fast(int32_t t, T (&f)[Q], T& rho, T& ux
     }
 }
 
+// Body force on FLUID cells: f_i += 3 w_i (e_i . g) after the collision (a
+// first-order forcing term; no counterpart in the reference, and a zero g
+// leaves every value unchanged).
+template <class T>
+__device__ __forceinline__ void add_force(int32_t t, T (&f)[Q], T gx, T gy, T gz) {
+    if (t != FLUID) return;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = f[i] + (T(3) * w_of<T>(i)) * edot<T>(i, gx, gy, gz);
+}
+
 template <class T, bool FAST>
 __device__ __forceinline__ void collide_any(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
                                             const StepConsts<T>& k) {
@@ -350,6 +360,7 @@ template <class T> struct StepArgs {
     StepConsts<T> k;
     float scale, inv_scale;
     T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
+    const T* force;       // body force g, 3 N^3 [gx | gy | gz], or null
 };
 
 // ---------------------------------------------------------------------------
@@ -409,6 +420,10 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
             f[i] = codec.load(src + base + off, w_of<T>(i));
         }
         collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        if (a.force) {
+            const int64_t c = int64_t(x) + int64_t(n) * r;
+            add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+        }
         const int64_t c0 = int64_t(n) * r;
         const int64_t obase = 19 * c0 - 18 * (c0 & smask);
 #pragma unroll
@@ -570,6 +585,10 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         for (int i = 0; i < Q; ++i) f[i] = codec.decode(g[i][j], w_of<T>(i));
         T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
         collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
+        if (a.force) {
+            const int64_t c = int64_t(x0 + j) + int64_t(n) * r;
+            add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+        }
         const bool store = is_store_macro(t);
         const T qnan = T(__builtin_nan(""));
         rho_v[j] = store ? rho : qnan;
@@ -672,6 +691,7 @@ __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a
             f[i] = codec.load(src + slot(s, i, a.stride), w_of<T>(i));
         }
         collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) codec.store(dst + slot(c, i, a.stride), w_of<T>(i), f[i]);
         if (a.update_macro && is_store_macro(t)) {
@@ -792,7 +812,7 @@ __global__ __launch_bounds__(256) void lbm_view_kernel(ViewArgs<T> a) {
 
 // Final reduction of the per-block (mass, energy) partials, in a fixed order:
 // thread j sums blocks j, j + 1024, ... then a tree over the 1024 threads.
-__global__ __launch_bounds__(1024) void lbm_reduce_partials(const double* partials, int64_t nblocks, double* out) {
+static __global__ __launch_bounds__(1024) void lbm_reduce_partials(const double* partials, int64_t nblocks, double* out) {
     __shared__ double sm[1024];
     __shared__ double se[1024];
     double m = 0.0, e = 0.0;

@@ -2,8 +2,10 @@
 //
 // Used by bench.py for two things: the sustained copy bandwidth of the GPU it
 // runs on (read + write bytes / kernel time, HIP events on the kernel's own
-// stream), and as the calibration point of the rocprofv3 FETCH_SIZE /
-// WRITE_SIZE counters (a copy of B bytes must fetch and write B bytes).
+// stream), which is the practical ceiling for any streaming kernel, and as the
+// calibration point of the rocprofv3 FETCH_SIZE / WRITE_SIZE counters (a copy
+// of B bytes must fetch and write B bytes).  One 16-byte element per thread
+// and non-temporal stores: the fastest copy shape measured on MI355X.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -12,8 +14,8 @@ namespace {
 
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                    int64_t n) {
-    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
         const uint4 v = src[i];
         __builtin_nontemporal_store(v.x, &dst[i].x);
         __builtin_nontemporal_store(v.y, &dst[i].y);
@@ -26,10 +28,13 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
 
 extern "C" {
 
-// Copy `bytes` (a multiple of 16) `reps` times between two fresh device
-// buffers; on success ms[r] holds the duration of repetition r.
+// Copy `bytes` (a multiple of 16, at most 2^31 * 256 * 16) `reps` times between
+// two fresh device buffers; on success ms[r] holds the duration of repetition r.
 int membw_copy(int device, int64_t bytes, int reps, double* ms) {
     if (bytes <= 0 || (bytes % 16) != 0 || reps <= 0 || !ms) return -1;
+    const int64_t n = bytes / 16;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return -1;
     if (hipSetDevice(device) != hipSuccess) return -2;
     void *a = nullptr, *b = nullptr;
     hipStream_t s = nullptr;
@@ -41,11 +46,9 @@ int membw_copy(int device, int64_t bytes, int reps, double* ms) {
         rc = -3;
     }
     if (!rc && hipMemsetAsync(a, 1, size_t(bytes), s) != hipSuccess) rc = -4;
-    const int64_t n = bytes / 16;
-    const int blocks = 256 * 64;  // persistent-style grid, grid-stride loop
     for (int r = 0; !rc && r < reps; ++r) {
         if (hipEventRecord(e0, s) != hipSuccess) rc = -4;
-        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4*>(a),
+        hipLaunchKernelGGL(copy_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<const uint4*>(a),
                            static_cast<uint4*>(b), n);
         if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
             hipEventSynchronize(e1) != hipSuccess)
