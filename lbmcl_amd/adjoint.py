@@ -119,6 +119,8 @@ def main(argv=None):
     ap.add_argument("--plane-z", type=int, default=None)
     ap.add_argument("--checkpoint-every", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--fd-directions", type=int, default=0,
+                    help="also compare with FP64 central differences (h = 1e-6 relative) on this many directions")
     a = ap.parse_args(argv)
     g = gaussian_random_field(a.dim, 8.0, 1e-6, a.seed)
     r = cavity_gradient(a.dim, a.steps, force=g, objective=a.objective, plane_z=a.plane_z, precision=a.precision,
@@ -127,6 +129,21 @@ def main(argv=None):
     r["d_force_norm"] = float(np.linalg.norm(dg))
     r["adjoint_over_forward"] = r["adjoint_ms"] / r["forward_ms"]
     r.update(dim=a.dim, steps=a.steps, precision=a.precision, objective=a.objective)
+    if a.fd_directions:
+        rng = np.random.default_rng(a.seed + 100)
+        U, nu, h = 0.05, 0.0089, 1e-6
+        errs = []
+        for _ in range(a.fd_directions):
+            dU, dnu = rng.standard_normal(), rng.standard_normal()
+            d = gaussian_random_field(a.dim, 8.0, 1.0, int(rng.integers(1 << 30)))
+            jp = cavity_objective(a.dim, a.steps, U + h * U * dU, nu + h * nu * dnu, g + h * 1e-6 * d,
+                                  a.objective, a.plane_z, "f64")
+            jm = cavity_objective(a.dim, a.steps, U - h * U * dU, nu - h * nu * dnu, g - h * 1e-6 * d,
+                                  a.objective, a.plane_z, "f64")
+            fd = (jp - jm) / (2 * h)
+            adj = r["d_velocity"] * U * dU + r["d_viscosity"] * nu * dnu + 1e-6 * float(np.sum(dg * d))
+            errs.append(abs(adj - fd) / abs(fd))
+        r["fd_relative_errors_vs_fp64"] = errs
     print(json.dumps(r))
     return 0
 
