@@ -42,16 +42,16 @@ def gaussian_random_field(n, correlation=8.0, amplitude=1e-6, seed=0, components
     return out
 
 
-def _context(dim, velocity, viscosity, precision, stride):
+def _context(dim, velocity, viscosity, precision, stride, kernel=0):
     return capi.Context(dim, viscosity=viscosity, velocity=velocity, stride=stride or dim, precision=precision,
-                        profile=False, raw_parameters=True)
+                        profile=False, raw_parameters=True, kernel=kernel)
 
 
 def cavity_objective(dim, steps, velocity=0.05, viscosity=0.0089, force=None, objective="kinetic_energy",
-                     plane_z=None, precision="f64", stride=None):
+                     plane_z=None, precision="f64", stride=None, kernel=0):
     """J by a plain forward run, evaluated in FP64 from the macroscopic fields."""
     plane_z = dim // 2 if plane_z is None else plane_z
-    with _context(dim, velocity, viscosity, precision, stride) as g:
+    with _context(dim, velocity, viscosity, precision, stride, kernel) as g:
         if force is not None:
             g.set_force(np.asarray(force).reshape(-1))
         g.initialize()
@@ -68,10 +68,14 @@ def cavity_objective(dim, steps, velocity=0.05, viscosity=0.0089, force=None, ob
 
 
 def cavity_gradient(dim, steps, velocity=0.05, viscosity=0.0089, force=None, objective="kinetic_energy",
-                    plane_z=None, precision="f64", stride=None, checkpoint_every=0, grad_force=True):
-    """dict: objective J, d_velocity, d_viscosity, d_force (3, dim^3), timings and memory."""
+                    plane_z=None, precision="f64", stride=None, checkpoint_every=0, grad_force=True, kernel=0):
+    """dict: objective J, d_velocity, d_viscosity, d_force (3, dim^3), timings and memory.
+
+    kernel: capi.LBM_KERNEL_* of the context; LBM_KERNEL_GENERIC or LBM_KERNEL_ROW
+    select the generic (one thread per cell) adjoint kernel, anything else the
+    vector adjoint kernel where the lattice allows it."""
     plane_z = dim // 2 if plane_z is None else plane_z
-    with _context(dim, velocity, viscosity, precision, stride) as g:
+    with _context(dim, velocity, viscosity, precision, stride, kernel) as g:
         if force is not None:
             g.set_force(np.asarray(force).reshape(-1))
         return g.adjoint(steps, OBJECTIVES[objective], plane_z, checkpoint_every, grad_force)
@@ -128,7 +132,8 @@ def main(argv=None):
     dg = r.pop("d_force")
     r["d_force_norm"] = float(np.linalg.norm(dg))
     r["adjoint_over_forward"] = r["adjoint_ms"] / r["forward_ms"]
-    r.update(dim=a.dim, steps=a.steps, precision=a.precision, objective=a.objective)
+    r["J"] = r.pop("objective")
+    r.update(dim=a.dim, steps=a.steps, precision=a.precision, objective_kind=a.objective)
     if a.fd_directions:
         rng = np.random.default_rng(a.seed + 100)
         U, nu, h = 0.05, 0.0089, 1e-6

@@ -34,6 +34,15 @@ struct AdjointBuffers {
     }
 };
 
+// The vector adjoint kernel needs whole wavefronts per row (N / V >= 64), at
+// most 512 threads per row, and V consecutive cells contiguous (stride >= V).
+// LBM_KERNEL_GENERIC and LBM_KERNEL_ROW select the generic adjoint kernel.
+template <class T> bool adjoint_vec_possible(const lbm_ctx* c) {
+    const int64_t V = lbm::AdjVec<T>::V;
+    if (c->cfg.kernel == LBM_KERNEL_GENERIC || c->cfg.kernel == LBM_KERNEL_ROW) return false;
+    return c->n / V >= 64 && c->n / V <= 512 && c->stride >= V;
+}
+
 template <class T>
 int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* res, double* grad_g_host) {
     const int64_t T_steps = cfg.steps;
@@ -74,8 +83,11 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         device_bytes += double(3 * c->n3) * sizeof(double);
         LBM_HIP_CHECK(hipMemsetAsync(grad_g, 0, size_t(3 * c->n3) * sizeof(double), c->stream), "hipMemsetAsync");
     }
-    const int block = 256;
-    const int64_t nblocks = (c->n3 + block - 1) / block;
+    // vector adjoint kernel: one block of N / V threads per interior row
+    const bool vec = adjoint_vec_possible<T>(c);
+    const int64_t V = lbm::AdjVec<T>::V;
+    const int block = vec ? int(c->n / V) : 256;
+    const int64_t nblocks = vec ? int64_t(c->n - 2) * (c->n - 2) : (c->n3 + 255) / 256;
     double* partials = nullptr;
     double* acc = nullptr;
     {
@@ -83,7 +95,10 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         if (rc) return rc;
         rc = buf.alloc(reinterpret_cast<void**>(&acc), 3 * sizeof(double), "accumulators");
         if (rc) return rc;
+        LBM_HIP_CHECK(hipMemsetAsync(partials, 0, size_t(3 * nblocks) * sizeof(double), c->stream), "hipMemsetAsync");
         LBM_HIP_CHECK(hipMemsetAsync(acc, 0, 3 * sizeof(double), c->stream), "hipMemsetAsync");
+        // slots that no step writes (fed by wall cells) must read as zero
+        for (int j = 0; j < 2; ++j) LBM_HIP_CHECK(hipMemsetAsync(lam[j], 0, S, c->stream), "hipMemsetAsync");
     }
 
     hipEvent_t e0, e1, e2;
@@ -138,6 +153,7 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
     a.U = T(c->U);
     a.objective = cfg.objective;
     a.plane_z = cfg.plane_z;
+    a.xcd_remap = c->cfg.xcd_remap;
     {
         // number of FLUID/MOVING cells in the plane z = plane_z (all interior cells of that plane but the
         // boundary-layer ones); counted on the host
@@ -153,10 +169,11 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         a.Lprev = static_cast<T*>(Lprev);
         a.seed = seed;
         a.first = first;
-        hipLaunchKernelGGL((lbm::lbm_adjoint_step<T>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
+        if (vec)
+            hipLaunchKernelGGL((lbm::lbm_adjoint_vec<T>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((lbm::lbm_adjoint_step<T>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
         LBM_HIP_CHECK(hipGetLastError(), "adjoint step launch");
-        hipLaunchKernelGGL(lbm::lbm_adjoint_reduce, dim3(1), dim3(1024), 0, c->stream, partials, nblocks, acc);
-        LBM_HIP_CHECK(hipGetLastError(), "adjoint reduce launch");
         return LBM_OK;
     };
 
@@ -183,6 +200,8 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
             cur = 1 - cur;
         }
     }
+    hipLaunchKernelGGL(lbm::lbm_adjoint_reduce, dim3(1), dim3(1024), 0, c->stream, partials, nblocks, acc);
+    LBM_HIP_CHECK(hipGetLastError(), "adjoint reduce launch");
     LBM_HIP_CHECK(hipEventRecord(e2, c->stream), "hipEventRecord");
 
     double h[3];

@@ -46,6 +46,7 @@ template <class T> struct AdjArgs {
     int32_t first;     // 1: the step that reads the initial state P_0 (adds dP_0/dU)
     int32_t objective;
     int32_t plane_z;
+    int32_t xcd_remap;
     double plane_scale;  // 1 / number of cells averaged by OBJ_MEAN_UX_PLANE
 };
 
@@ -65,29 +66,175 @@ __device__ __forceinline__ int lid_multiplicity(int i) {
                                                                    : 1;
 }
 
-__device__ __forceinline__ void block_reduce3(double& a, double& b, double& c, double* lds) {
-    // serial LDS sum (deterministic for any block size)
-    __syncthreads();
-    lds[3 * threadIdx.x] = a;
-    lds[3 * threadIdx.x + 1] = b;
-    lds[3 * threadIdx.x + 2] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sa = 0.0, sb = 0.0, sc = 0.0;
-        for (uint32_t j = 0; j < blockDim.x; ++j) {
-            sa += lds[3 * j];
-            sb += lds[3 * j + 1];
-            sc += lds[3 * j + 2];
+// Local vector-Jacobian product of one non-wall cell of type t in plane z.
+// f: the populations the cell gathered (the lid fix-up is applied here); L:
+// lambda_{k+1} at the cell's own slots (unused when seeding).  Writes lam
+// (dJ/dG_k of the gathered populations) and the cell's parameter terms:
+// dU and dW (d/domega) and J are set, gg = d/dg of the cell.
+template <class T>
+__device__ __forceinline__ void adjoint_cell(const AdjArgs<T>& a, int32_t t, int32_t z, T (&f)[Q], const T (&L)[Q],
+                                             T (&lam)[Q], double& dU, double& dW, double& J, double (&gg)[3]) {
+    dU = dW = J = 0.0;
+    gg[0] = gg[1] = gg[2] = 0.0;
+    const bool moving = is_moving(t);
+    const T U = a.U;
+    if (moving) {  // lid fix-up (kernels.cl:343-349)
+        f[5] = f[6];
+        f[11] = f[17];
+        f[12] = f[18];
+        f[13] = f[15];
+        f[14] = f[16];
+    }
+    T rho = T(0), jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        rho += f[i];
+        jx += T(ex_of(i)) * f[i];
+        jy += T(ey_of(i)) * f[i];
+        jz += T(ez_of(i)) * f[i];
+    }
+    const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    if (a.seed) {
+        // lambda_T = dJ / dG_T for the objective
+        bool any = false;
+        if (a.objective == OBJ_KINETIC_ENERGY && is_store_macro(t)) {
+            if (moving) {
+                J = 0.5 * double(rho) * double(U) * double(U);
+                dU = double(rho) * double(U);
+#pragma unroll
+                for (int i = 0; i < Q; ++i) lam[i] = T(lid_multiplicity(i)) * (T(0.5) * U * U);
+            } else {
+                const T u2 = ux * ux + uy * uy + uz * uz;
+                J = 0.5 * double(rho) * double(u2);
+#pragma unroll
+                for (int i = 0; i < Q; ++i) lam[i] = edot<T>(i, ux, uy, uz) - T(0.5) * u2;
+            }
+            any = true;
+        } else if (a.objective == OBJ_MEAN_UX_PLANE && is_store_macro(t) && z == a.plane_z) {
+            const T sc = T(a.plane_scale);
+            if (moving) {
+                J = a.plane_scale * double(U);
+                dU = a.plane_scale;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) lam[i] = T(0);
+            } else {
+                J = a.plane_scale * double(ux);
+#pragma unroll
+                for (int i = 0; i < Q; ++i) lam[i] = sc * (T(ex_of(i)) - ux) / rho;
+            }
+            any = true;
         }
-        a = sa;
-        b = sb;
-        c = sc;
+        if (!any) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) lam[i] = T(0);
+        }
+        return;
+    }
+    const T om = a.inv_tau;
+    if (t == FLUID) {
+        const T u2 = ux * ux + uy * uy + uz * uz;
+        T A = T(0), Bx = T(0), By = T(0), Bz = T(0), dw = T(0);
+        double ggx = 0.0, ggy = 0.0, ggz = 0.0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const T eu = edot<T>(j, ux, uy, uz);
+            const T w = w_of<T>(j);
+            const T feq = rho * w * (T(1) + T(3) * eu + T(4.5) * eu * eu - T(1.5) * u2);
+            A += L[j] * feq;
+            const T q = L[j] * rho * w;
+            const T k9 = T(9) * eu;
+            Bx += q * ((T(3) + k9) * T(ex_of(j)) - T(3) * ux);
+            By += q * ((T(3) + k9) * T(ey_of(j)) - T(3) * uy);
+            Bz += q * ((T(3) + k9) * T(ez_of(j)) - T(3) * uz);
+            dw += L[j] * (feq - f[j]);
+            const double lw = 3.0 * double(L[j]) * double(w);
+            ggx += lw * double(ex_of(j));
+            ggy += lw * double(ey_of(j));
+            ggz += lw * double(ez_of(j));
+        }
+        A = A / rho;
+        const T Bu = Bx * ux + By * uy + Bz * uz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lam[i] = (T(1) - om) * L[i] + om * (A + (edot<T>(i, Bx, By, Bz) - Bu) / rho);
+        dW = double(dw);
+        gg[0] = ggx;
+        gg[1] = ggy;
+        gg[2] = ggz;
+    } else if (moving) {
+        T S = T(0), dUl = T(0);
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const T ex = T(ex_of(j));
+            const T w = w_of<T>(j);
+            const T cj = T(1) + T(3) * ex * U + T(4.5) * ex * ex * U * U - T(1.5) * U * U;
+            S += L[j] * w * cj;
+            dUl += L[j] * rho * w * ((T(3) * ex + T(9) * ex * ex * U) - T(3) * U);
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lam[i] = T(lid_multiplicity(i)) * S;
+        dU = double(dUl);
+    } else if (is_bounceback(t)) {
+        constexpr int OPPI[Q] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15, 16, 13, 14, 11, 12};
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lam[i] = L[OPPI[i]];
+    } else {  // corner
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lam[i] = L[i];
     }
 }
 
+// dJ/dU through the gathers of cell (x, y, z) that read initial values: from
+// wall cells, and at the first step from every cell.  Only lid-layer cells
+// (FRONT) gather initial values that depend on U.
+template <class T>
+__device__ __forceinline__ void adjoint_initial_sources(const AdjArgs<T>& a, int32_t t, int32_t x, int32_t y,
+                                                        int32_t z, const T (&lam)[Q], double& dU) {
+    if (!(t & FRONT)) return;
+    const int32_t n = a.n;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const bool wall_src = wall_coord(x - ex_of(i), y - ey_of(i), z - ez_of(i), n);
+        if (wall_src || a.first) dU += double(lam[i]) * double(dF0_dU<T>(i, a.U));
+    }
+}
+
+// Block sums of (dU, dW, J) in FP64 added to the block's own partials slot
+// (one slot per block, accumulated over all steps of the sweep; the order of
+// every sum is fixed).  Blocks are whole wavefronts.
+__device__ __forceinline__ void block_accumulate3(double a, double b, double c, double* lds, double* slot) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    if (lane == 0) {
+        lds[3 * wave] = a;
+        lds[3 * wave + 1] = b;
+        lds[3 * wave + 2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = lds[0], sb = lds[1], sc = lds[2];
+        for (int w = 1; w < nwaves; ++w) {
+            sa += lds[3 * w];
+            sb += lds[3 * w + 1];
+            sc += lds[3 * w + 2];
+        }
+        slot[0] += sa;
+        slot[1] += sb;
+        slot[2] += sc;
+    }
+}
+
+// Generic adjoint step: one thread per cell of a 1-D grid over all N^3 cells
+// (block size 256), 64-bit slot arithmetic per population.
 template <class T>
 __global__ __launch_bounds__(256) void lbm_adjoint_step(AdjArgs<T> a) {
-    __shared__ double lds[3 * 256];
+    __shared__ double lds[3 * 4];
     const int32_t n = a.n;
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool inside = c < a.n3;
@@ -97,146 +244,206 @@ __global__ __launch_bounds__(256) void lbm_adjoint_step(AdjArgs<T> a) {
     const int32_t t = inside ? cell_type(x, y, z, n) : WALL;
     double dU = 0.0, dW = 0.0, J = 0.0;
     if (t != WALL) {
-        T f[Q];
+        T f[Q], L[Q], lam[Q];
+        double gg[3];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
             f[i] = a.P[slot(s, i, a.stride)];
         }
-        T lam[Q];
-        const bool moving = is_moving(t);
-        const T U = a.U;
-        if (moving) {  // lid fix-up (kernels.cl:343-349)
-            f[5] = f[6];
-            f[11] = f[17];
-            f[12] = f[18];
-            f[13] = f[15];
-            f[14] = f[16];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) L[j] = a.seed ? T(0) : a.Lnext[slot(c, j, a.stride)];
+        adjoint_cell<T>(a, t, z, f, L, lam, dU, dW, J, gg);
+        if (a.grad_g && t == FLUID && !a.seed) {
+            a.grad_g[c] += gg[0];
+            a.grad_g[a.n3 + c] += gg[1];
+            a.grad_g[2 * a.n3 + c] += gg[2];
         }
-        T rho = T(0), jx = T(0), jy = T(0), jz = T(0);
+        adjoint_initial_sources<T>(a, t, x, y, z, lam, dU);
+        // scatter to the gather positions; own slots whose destination is a
+        // wall are read by nobody and get zero
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            rho += f[i];
-            jx += T(ex_of(i)) * f[i];
-            jy += T(ey_of(i)) * f[i];
-            jz += T(ez_of(i)) * f[i];
-        }
-        const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
-        if (a.seed) {
-            // lambda_T = dJ / dG_T for the objective
-            bool any = false;
-            if (a.objective == OBJ_KINETIC_ENERGY && is_store_macro(t)) {
-                if (moving) {
-                    J = 0.5 * double(rho) * double(U) * double(U);
-                    dU = double(rho) * double(U);
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) lam[i] = T(lid_multiplicity(i)) * (T(0.5) * U * U);
-                } else {
-                    const T u2 = ux * ux + uy * uy + uz * uz;
-                    J = 0.5 * double(rho) * double(u2);
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) lam[i] = edot<T>(i, ux, uy, uz) - T(0.5) * u2;
-                }
-                any = true;
-            } else if (a.objective == OBJ_MEAN_UX_PLANE && is_store_macro(t) && z == a.plane_z) {
-                const T sc = T(a.plane_scale);
-                if (moving) {
-                    J = a.plane_scale * double(U);
-                    dU = a.plane_scale;
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) lam[i] = T(0);
-                } else {
-                    J = a.plane_scale * double(ux);
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) lam[i] = sc * (T(ex_of(i)) - ux) / rho;
-                }
-                any = true;
-            }
-            if (!any) {
-#pragma unroll
-                for (int i = 0; i < Q; ++i) lam[i] = T(0);
-            }
-        } else {
-            T L[Q];
-#pragma unroll
-            for (int j = 0; j < Q; ++j) L[j] = a.Lnext[slot(c, j, a.stride)];
-            const T om = a.inv_tau;
-            if (t == FLUID) {
-                const T u2 = ux * ux + uy * uy + uz * uz;
-                T A = T(0), Bx = T(0), By = T(0), Bz = T(0), dw = T(0);
-                double ggx = 0.0, ggy = 0.0, ggz = 0.0;
-#pragma unroll
-                for (int j = 0; j < Q; ++j) {
-                    const T eu = edot<T>(j, ux, uy, uz);
-                    const T w = w_of<T>(j);
-                    const T feq = rho * w * (T(1) + T(3) * eu + T(4.5) * eu * eu - T(1.5) * u2);
-                    A += L[j] * feq;
-                    const T q = L[j] * rho * w;
-                    const T k9 = T(9) * eu;
-                    Bx += q * ((T(3) + k9) * T(ex_of(j)) - T(3) * ux);
-                    By += q * ((T(3) + k9) * T(ey_of(j)) - T(3) * uy);
-                    Bz += q * ((T(3) + k9) * T(ez_of(j)) - T(3) * uz);
-                    dw += L[j] * (feq - f[j]);
-                    const double lw = 3.0 * double(L[j]) * double(w);
-                    ggx += lw * double(ex_of(j));
-                    ggy += lw * double(ey_of(j));
-                    ggz += lw * double(ez_of(j));
-                }
-                A = A / rho;
-                const T Bu = Bx * ux + By * uy + Bz * uz;
-#pragma unroll
-                for (int i = 0; i < Q; ++i)
-                    lam[i] = (T(1) - om) * L[i] + om * (A + (edot<T>(i, Bx, By, Bz) - Bu) / rho);
-                dW = double(dw);
-                if (a.grad_g) {
-                    a.grad_g[c] += ggx;
-                    a.grad_g[a.n3 + c] += ggy;
-                    a.grad_g[2 * a.n3 + c] += ggz;
-                }
-            } else if (moving) {
-                T S = T(0), dUl = T(0);
-#pragma unroll
-                for (int j = 0; j < Q; ++j) {
-                    const T ex = T(ex_of(j));
-                    const T w = w_of<T>(j);
-                    const T cj = T(1) + T(3) * ex * U + T(4.5) * ex * ex * U * U - T(1.5) * U * U;
-                    S += L[j] * w * cj;
-                    dUl += L[j] * rho * w * ((T(3) * ex + T(9) * ex * ex * U) - T(3) * U);
-                }
-#pragma unroll
-                for (int i = 0; i < Q; ++i) lam[i] = T(lid_multiplicity(i)) * S;
-                dU = double(dUl);
-            } else if (is_bounceback(t)) {
-                constexpr int OPPI[Q] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15, 16, 13, 14, 11, 12};
-#pragma unroll
-                for (int i = 0; i < Q; ++i) lam[i] = L[OPPI[i]];
-            } else {  // corner
-#pragma unroll
-                for (int i = 0; i < Q; ++i) lam[i] = L[i];
-            }
-        }
-        // scatter to the gather positions; gathers from walls and (first step)
-        // from the initial lattice depend on U through F0[d][i]
-        const bool front = (t & FRONT) != 0;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const int32_t sx = x - ex_of(i), sy = y - ey_of(i), sz = z - ez_of(i);
-            const bool wall_src = wall_coord(sx, sy, sz, n);
-            if (front && (wall_src || a.first)) dU += double(lam[i]) * double(dF0_dU<T>(i, U));
-            if (!wall_src) {
+            if (!wall_coord(x - ex_of(i), y - ey_of(i), z - ez_of(i), n)) {
                 const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
                 a.Lprev[slot(s, i, a.stride)] = lam[i];
             }
-            const int32_t dx = x + ex_of(i), dy = y + ey_of(i), dz = z + ez_of(i);
-            if (wall_coord(dx, dy, dz, n)) a.Lprev[slot(c, i, a.stride)] = T(0);
+            if (wall_coord(x + ex_of(i), y + ey_of(i), z + ez_of(i), n)) a.Lprev[slot(c, i, a.stride)] = T(0);
         }
     }
-    block_reduce3(dU, dW, J, lds);
-    if (threadIdx.x == 0) {
-        a.partials[3 * blockIdx.x] = dU;
-        a.partials[3 * blockIdx.x + 1] = dW;
-        a.partials[3 * blockIdx.x + 2] = J;
+    block_accumulate3(dU, dW, J, lds, a.partials + 3 * blockIdx.x);
+}
+
+// Vector adjoint step (FP32: 2 cells per lane, FP64: 1).  One block per
+// interior row (y, z), N / V threads, so every cross-lane move stays inside
+// the block.  The gather of P_k is the forward vector kernel's (aligned 8-byte
+// loads, DPP shifts by one lane, wavefront-uniform loads at the wavefront
+// ends); lambda_{k+1} is read at the lane's own aligned slots.  The scatter
+// lambda_k[d - e_i][i] = lam_i(d) is the gather reversed: the vector written at
+// the lane's aligned position of population i's source row holds lam_i of the
+// cells x + e_x, moved by one lane the other way, and the element that crosses
+// a wavefront end comes from the neighbouring wavefront through LDS.  Wall
+// cells of the row scatter zeros, which is what the slots they feed must hold;
+// slots fed by wall rows are never written and keep the zeros both adjoint
+// lattices start with.
+template <class T> struct AdjVec {
+    static constexpr int V = int(8 / sizeof(T));
+};
+
+template <class T>
+__global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
+    constexpr int V = AdjVec<T>::V;
+    typedef T SV __attribute__((ext_vector_type(V)));
+    __shared__ T xup[8][5];  // per wavefront: lane 0's lam_i of its first cell, e_x(i) = +1
+    __shared__ T xdn[8][5];  // per wavefront: lane 63's lam_i of its last cell, e_x(i) = -1
+    __shared__ double lds[3 * 8];
+    const int32_t n = a.n;
+    const uint32_t rows = uint32_t(n - 2);
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    const int32_t y = int32_t(lb % rows) + 1;
+    const int32_t z = int32_t(lb / rows) + 1;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    const uint32_t nwaves = blockDim.x >> 6;
+    const int32_t x0 = int32_t(tid * V);
+    const int32_t xw = __builtin_amdgcn_readfirstlane(int32_t((tid & ~63u) * V));
+    const int64_t r = int64_t(y) + int64_t(n) * z;
+    const int32_t rf = row_flags(y, z, n);
+    const int64_t smask = a.stride - 1;
+    const uint32_t smask32 = uint32_t(smask);
+    auto voff = [&](int32_t xx) -> uint32_t {  // v(x) = slot(N r + x) - slot(N r)
+        const uint32_t u = uint32_t(xx);
+        return 19u * u - 18u * (u & smask32);
+    };
+    auto row_slot = [&](int i) -> int64_t {  // slot 0 of population i's source row (= its scatter row)
+        const int64_t c0 = int64_t(n) * (r - ey_of(i) - int64_t(n) * ez_of(i));
+        return 19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride;
+    };
+    const int64_t oc0 = int64_t(n) * r;
+    const int64_t own = 19 * oc0 - 18 * (oc0 & smask);
+    const uint32_t off0 = voff(x0);
+    const bool has_left = xw > 0;
+    const bool has_right = xw + 64 * V < n;
+
+    SV A[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) A[i] = *reinterpret_cast<const SV*>(a.P + row_slot(i) + off0);
+    SV Lv[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+        if (a.seed) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) Lv[j][k] = T(0);
+        } else {
+            Lv[j] = *reinterpret_cast<const SV*>(a.Lnext + own + int64_t(j) * a.stride + off0);
+        }
     }
+    T edge[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        edge[i] = T(0);
+        if (ex_of(i) == 1 && has_left) edge[i] = a.P[row_slot(i) + voff(xw - 1)];
+        if (ex_of(i) == -1 && has_right) edge[i] = a.P[row_slot(i) + voff(xw + 64 * V)];
+    }
+    T g[Q][V];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int ex = ex_of(i);
+        if (ex == 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) g[i][j] = A[i][j];
+        } else if (ex == 1) {
+            g[i][0] = lane_shift<T>(A[i][V - 1], edge[i], true);
+#pragma unroll
+            for (int j = 1; j < V; ++j) g[i][j] = A[i][j - 1];
+        } else {
+#pragma unroll
+            for (int j = 0; j < V - 1; ++j) g[i][j] = A[i][j + 1];
+            g[i][V - 1] = lane_shift<T>(A[i][0], edge[i], false);
+        }
+    }
+
+    T lam[Q][V];
+    double dU = 0.0, dW = 0.0, J = 0.0;
+    double gv[3][V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int32_t t = finish_type(x_flags(x0 + j, n) | rf);
+        T f[Q], L[Q], lc[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            f[i] = g[i][j];
+            L[i] = Lv[i][j];
+            lc[i] = T(0);
+        }
+        double cu = 0.0, cw = 0.0, cj = 0.0, gg[3] = {0.0, 0.0, 0.0};
+        if (t != WALL) {
+            adjoint_cell<T>(a, t, z, f, L, lc, cu, cw, cj, gg);
+            adjoint_initial_sources<T>(a, t, x0 + j, y, z, lc, cu);
+        }
+        dU += cu;
+        dW += cw;
+        J += cj;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lam[i][j] = lc[i];
+        const bool gfl = (t == FLUID) && !a.seed;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gv[k][j] = gfl ? gg[k] : 0.0;
+    }
+    if (a.grad_g && !a.seed) {
+        typedef double DV __attribute__((ext_vector_type(V)));
+        const int64_t c = int64_t(x0) + oc0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            DV* p = reinterpret_cast<DV*>(a.grad_g + int64_t(k) * a.n3 + c);
+            DV v = *p;
+#pragma unroll
+            for (int j = 0; j < V; ++j) v[j] += gv[k][j];
+            *p = v;
+        }
+    }
+
+    // wavefront-end elements of the scatter through LDS
+    {
+        int ku = 0, kd = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (ex_of(i) == 1) {
+                if (lane == 0) xup[wave][ku] = lam[i][0];
+                ++ku;
+            } else if (ex_of(i) == -1) {
+                if (lane == 63) xdn[wave][kd] = lam[i][V - 1];
+                ++kd;
+            }
+        }
+    }
+    __syncthreads();
+    {
+        int ku = 0, kd = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int ex = ex_of(i);
+            SV o;
+            if (ex == 0) {
+#pragma unroll
+                for (int j = 0; j < V; ++j) o[j] = lam[i][j];
+            } else if (ex == 1) {  // position x holds lam_i(x + 1)
+                const T e = (wave + 1 < nwaves) ? xup[wave + 1][ku] : T(0);
+#pragma unroll
+                for (int j = 0; j < V - 1; ++j) o[j] = lam[i][j + 1];
+                o[V - 1] = lane_shift<T>(lam[i][0], e, false);
+                ++ku;
+            } else {  // position x holds lam_i(x - 1)
+                const T e = (wave > 0) ? xdn[wave - 1][kd] : T(0);
+                o[0] = lane_shift<T>(lam[i][V - 1], e, true);
+#pragma unroll
+                for (int j = 1; j < V; ++j) o[j] = lam[i][j - 1];
+                ++kd;
+            }
+            __builtin_nontemporal_store(o, reinterpret_cast<SV*>(a.Lprev + row_slot(i) + off0));
+        }
+    }
+    block_accumulate3(dU, dW, J, lds, a.partials + 3 * lb);
 }
 
 // acc[0..2] += sum of the per-block partials, in a fixed order
