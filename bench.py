@@ -156,6 +156,14 @@ def main():
                                                 "bytes_per_update", "steps")}
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
         variants["fp16_speedup_vs_f32"] = variants["f32_fp16"]["mlups"] / main_run["mlups"]
+        # the 128^3 configuration: MLUPS over iterations 100..1000 (99 untimed first)
+        for prec in ("f32", "f64"):
+            name = f"cavity_128_{prec}_it100_1000"
+            log(f"[bench] variant {name}")
+            r = run_lattice(128, 901, 99, prec, "native", 64, args.kernel, not args.no_remap)
+            variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
+                                                "bytes_per_update", "steps")}
+            log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
 
     log("[bench] copy bandwidth")
     copy = copy_bandwidth()

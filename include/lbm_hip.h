@@ -118,7 +118,8 @@ typedef struct lbm_ctx lbm_ctx;
 /* Number of HIP devices (CLUtil.hpp:145-178 lists devices). */
 int lbm_get_device_count(int32_t* count);
 
-/* Device name (replaces CL_DEVICE_NAME, lbmcl.hpp:626, 651). */
+/* Device name (replaces CL_DEVICE_NAME, lbmcl.hpp:626, 651): the HIP
+ * architecture name, e.g. "gfx950:sramecc+:xnack-", as ROCm's OpenCL reports it. */
 int lbm_get_device_name(int32_t device, char* name, size_t len);
 
 /* Create a context: select the device, create the stream, allocate the two
@@ -179,6 +180,36 @@ int lbm_read_state(lbm_ctx* ctx, int32_t buffer, void* out);
 /* Blocking write of lattice `buffer` in stored (pull) form; restores a state
  * read by lbm_read_state.  No counterpart in the reference. */
 int lbm_write_state(lbm_ctx* ctx, int32_t buffer, const void* in);
+
+/* Asynchronous snapshots for the output path.  They replace the blocking
+ * enqueueReadBuffer calls of the reference's dump writers (lbmcl.hpp:164, 211,
+ * 268, 274) where the caller formats files on other threads while the device
+ * keeps computing.
+ *
+ * lbm_snapshot enqueues, after all work enqueued so far on the context's
+ * stream, a pack of the requested data into a device staging slot and the
+ * copy of that slot into pinned host memory on a second stream, and returns a
+ * ticket without waiting.  The context has LBM_SNAPSHOT_SLOTS slots; when all
+ * are taken (enqueued, or waited and not yet released) lbm_snapshot blocks
+ * until another thread releases one.  lbm_snapshot_wait blocks until the copy
+ * of `ticket` is complete and returns its host data, valid until
+ * lbm_snapshot_release(ticket).  lbm_snapshot_wait and lbm_snapshot_release
+ * may be called from any thread; lbm_destroy waits for outstanding copies.
+ *
+ *   LBM_SNAPSHOT_MACRO  rho and u on the VTI window x, y, z in [1, dim-2]
+ *                       (x fastest, W = dim-2): W^3 rho values, then W^3
+ *                       (ux, uy, uz) triples; arithmetic type; `buffer` unused.
+ *   LBM_SNAPSHOT_F      lattice `buffer` in the push view of lbm_read_f, cell
+ *                       major: value (c, i) at c*19 + i, dim^3 cells.
+ *   LBM_SNAPSHOT_MAP    the dim^3 int32 cell flags of lbm_read_map.
+ * Packing is recorded as a READ_RHO, READ_F or READ_MAP event. */
+#define LBM_SNAPSHOT_MACRO 0
+#define LBM_SNAPSHOT_F 1
+#define LBM_SNAPSHOT_MAP 2
+#define LBM_SNAPSHOT_SLOTS 3
+int lbm_snapshot(lbm_ctx* ctx, int32_t what, int32_t buffer, int64_t* ticket);
+int lbm_snapshot_wait(lbm_ctx* ctx, int64_t ticket, const void** data, int64_t* bytes);
+int lbm_snapshot_release(lbm_ctx* ctx, int64_t ticket);
 
 /* Mass and kinetic energy of the last lbm_compute with diagnostics != 0
  * (blocking). */

@@ -35,8 +35,11 @@ EXPORTS = (
     "lbm_compute", "lbm_compute_range", "lbm_finish", "lbm_read_map", "lbm_read_rho", "lbm_read_u",
     "lbm_read_f", "lbm_read_state", "lbm_write_state", "lbm_read_diagnostics", "lbm_get_constants",
     "lbm_event_count", "lbm_event_get", "lbm_total_time_ms", "lbm_kernels_time_ms", "lbm_clear_events",
-    "lbm_set_force", "lbm_adjoint", "lbm_last_error",
+    "lbm_set_force", "lbm_adjoint", "lbm_snapshot", "lbm_snapshot_wait", "lbm_snapshot_release", "lbm_last_error",
 )
+
+LBM_SNAPSHOT_MACRO, LBM_SNAPSHOT_F, LBM_SNAPSHOT_MAP = 0, 1, 2
+LBM_SNAPSHOT_SLOTS = 3
 
 LBM_OBJECTIVE_KINETIC_ENERGY, LBM_OBJECTIVE_MEAN_UX_PLANE = 0, 1
 
@@ -130,6 +133,9 @@ def _declare(L):
         "lbm_clear_events": [vp],
         "lbm_set_force": [vp, vp],
         "lbm_adjoint": [vp, P(LbmAdjointConfig), P(LbmAdjointResult), vp],
+        "lbm_snapshot": [vp, i32, i32, P(i64)],
+        "lbm_snapshot_wait": [vp, i64, P(vp), P(i64)],
+        "lbm_snapshot_release": [vp, i64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -254,6 +260,24 @@ class Context:
         out = np.empty(19 * self.n3, dtype=self.dtype)
         check(lib().lbm_read_f(self._h, int(buffer), _ptr(out)), "lbm_read_f")
         return out
+
+    def snapshot(self, what, buffer=0):
+        """Enqueue an asynchronous snapshot (LBM_SNAPSHOT_*); returns its ticket."""
+        t = ctypes.c_int64(0)
+        check(lib().lbm_snapshot(self._h, int(what), int(buffer), ctypes.byref(t)), "lbm_snapshot")
+        return t.value
+
+    def snapshot_wait(self, ticket, what):
+        """Wait for a snapshot and return a copy of its data (the slot stays held)."""
+        p, nbytes = ctypes.c_void_p(), ctypes.c_int64(0)
+        check(lib().lbm_snapshot_wait(self._h, int(ticket), ctypes.byref(p), ctypes.byref(nbytes)),
+              "lbm_snapshot_wait")
+        dt = np.int32 if what == LBM_SNAPSHOT_MAP else self.dtype
+        buf = (ctypes.c_char * nbytes.value).from_address(p.value)
+        return np.frombuffer(buf, dtype=dt).copy()
+
+    def snapshot_release(self, ticket):
+        check(lib().lbm_snapshot_release(self._h, int(ticket)), "lbm_snapshot_release")
 
     def read_state(self, buffer):
         out = np.empty(19 * self.n3, dtype=self.sdtype)

@@ -8,6 +8,22 @@
 
 using namespace lbm_host;
 
+namespace {
+
+// The push view of lattice `buffer` into device memory `out` with CSoA stride
+// out_stride, on the context's stream.
+int enqueue_view(lbm_ctx* c, int32_t buffer, void* out, int64_t out_stride) {
+    if (c->cfg.precision == LBM_F64)
+        return launch_view<double, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<double*>(out), out_stride);
+    if (c->cfg.storage == LBM_STORE_FP16_SHIFT)
+        return launch_view<float, lbm::ST_FP16>(c, c->lat[buffer], static_cast<float*>(out), out_stride);
+    if (c->cfg.storage == LBM_STORE_BF16_SHIFT)
+        return launch_view<float, lbm::ST_BF16>(c, c->lat[buffer], static_cast<float*>(out), out_stride);
+    return launch_view<float, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<float*>(out), out_stride);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* lbm_last_error(void) { return g_last_error.c_str(); }
@@ -35,7 +51,10 @@ int lbm_get_device_name(int32_t device, char* name, size_t len) {
     }
     hipDeviceProp_t prop;
     LBM_HIP_CHECK(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    std::snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+    // the architecture name, which is what ROCm's OpenCL reports as
+    // CL_DEVICE_NAME for the same GPU (the marketing name can be empty when
+    // the system's GPU id table is missing)
+    std::snprintf(name, len, "%s", prop.gcnArchName);
     return LBM_OK;
 }
 
@@ -121,7 +140,6 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if ((rc = dev_alloc(&c->lat[0], lat_bytes, "lattice 0")) || (rc = dev_alloc(&c->lat[1], lat_bytes, "lattice 1")) ||
         (rc = dev_alloc(&c->rho, size_t(n3) * c->tsize, "rho")) ||
         (rc = dev_alloc(&c->u, size_t(3 * n3) * c->tsize, "u")) ||
-        (rc = dev_alloc(reinterpret_cast<void**>(&c->map), size_t(n3) * 4, "map")) ||
         (rc = dev_alloc(reinterpret_cast<void**>(&c->partials), size_t(c->partials_cap) * 2 * sizeof(double),
                         "partials")) ||
         (rc = dev_alloc(reinterpret_cast<void**>(&c->diag), 2 * sizeof(double), "diagnostics"))) {
@@ -203,7 +221,15 @@ int lbm_read_map(lbm_ctx* c, int32_t* out) {
     LBM_CHECK_CTX(c);
     LBM_CHECK_INIT(c);
     if (!out) return fail(LBM_ERR_INVALID, "null output");
-    return read_device(c, LBM_EVENT_READ_MAP, out, c->map, size_t(c->n3) * 4);
+    EventRec r;
+    int rc = ev_begin(c, LBM_EVENT_READ_MAP, r);
+    if (!rc) rc = ensure_map(c);
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipMemcpyAsync(out, c->map, size_t(c->n3) * 4, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync D2H");
+    rc = ev_end(c, r);
+    if (rc) return rc;
+    LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return LBM_OK;
 }
 
 int lbm_read_rho(lbm_ctx* c, void* out) {
@@ -231,16 +257,7 @@ int lbm_read_f(lbm_ctx* c, int32_t buffer, void* out) {
     if (rc) return rc;
     EventRec r;
     rc = ev_begin(c, LBM_EVENT_READ_F, r);
-    if (!rc) {
-        if (c->cfg.precision == LBM_F64)
-            rc = launch_view<double, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<double*>(tmp));
-        else if (c->cfg.storage == LBM_STORE_FP16_SHIFT)
-            rc = launch_view<float, lbm::ST_FP16>(c, c->lat[buffer], static_cast<float*>(tmp));
-        else if (c->cfg.storage == LBM_STORE_BF16_SHIFT)
-            rc = launch_view<float, lbm::ST_BF16>(c, c->lat[buffer], static_cast<float*>(tmp));
-        else
-            rc = launch_view<float, lbm::ST_NATIVE>(c, c->lat[buffer], static_cast<float*>(tmp));
-    }
+    if (!rc) rc = enqueue_view(c, buffer, tmp, c->stride);
     if (!rc) {
         const hipError_t e = hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) rc = fail(LBM_ERR_RUNTIME, std::string("hipMemcpyAsync D2H: ") + hipGetErrorString(e));
@@ -251,6 +268,134 @@ int lbm_read_f(lbm_ctx* c, int32_t buffer, void* out) {
     if (rc) return rc;
     if (se != hipSuccess) return fail(LBM_ERR_RUNTIME, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
     return LBM_OK;
+}
+
+int lbm_snapshot(lbm_ctx* c, int32_t what, int32_t buffer, int64_t* ticket) {
+    LBM_CHECK_CTX(c);
+    LBM_CHECK_INIT(c);
+    if (!ticket) return fail(LBM_ERR_INVALID, "null ticket");
+    size_t bytes = 0;
+    int kind = 0;
+    switch (what) {
+        case LBM_SNAPSHOT_MACRO:
+            bytes = size_t(4) * size_t(c->n - 2) * size_t(c->n - 2) * size_t(c->n - 2) * c->tsize;
+            kind = LBM_EVENT_READ_RHO;
+            break;
+        case LBM_SNAPSHOT_F:
+            if (buffer != 0 && buffer != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
+            bytes = size_t(c->n3) * lbm::Q * c->tsize;
+            kind = LBM_EVENT_READ_F;
+            break;
+        case LBM_SNAPSHOT_MAP:
+            bytes = size_t(c->n3) * 4;
+            kind = LBM_EVENT_READ_MAP;
+            break;
+        default:
+            return fail(LBM_ERR_INVALID, "unknown snapshot kind");
+    }
+    if (!c->copy_stream) LBM_HIP_CHECK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    SnapSlot* s = nullptr;
+    int64_t t = 0;
+    {
+        std::unique_lock<std::mutex> lk(c->snap_mu);
+        for (;;) {
+            for (SnapSlot& x : c->snaps)
+                if (x.state == 0) {
+                    s = &x;
+                    break;
+                }
+            if (s) break;
+            c->snap_cv.wait(lk);
+        }
+        t = c->next_ticket++;
+        s->ticket = t;
+        s->state = 1;
+    }
+    auto give_back = [&](int rc) {
+        std::lock_guard<std::mutex> lk(c->snap_mu);
+        s->state = 0;
+        s->ticket = -1;
+        c->snap_cv.notify_all();
+        return rc;
+    };
+    if (s->cap < bytes) {
+        if (s->dev) (void)hipFree(s->dev);
+        if (s->host) (void)hipHostFree(s->host);
+        s->dev = s->host = nullptr;
+        s->cap = 0;
+        int rc = dev_alloc(&s->dev, bytes, "snapshot staging");
+        if (rc) return give_back(rc);
+        if (hipHostMalloc(&s->host, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            s->host = nullptr;
+            return give_back(fail(LBM_ERR_MEMORY, "hipHostMalloc(snapshot, " + std::to_string(bytes) + " bytes) failed"));
+        }
+        s->cap = bytes;
+    }
+    if (!s->copied && hipEventCreateWithFlags(&s->copied, hipEventDisableTiming) != hipSuccess)
+        return give_back(fail(LBM_ERR_RUNTIME, "hipEventCreate failed"));
+    if (!s->packed && hipEventCreateWithFlags(&s->packed, hipEventDisableTiming) != hipSuccess)
+        return give_back(fail(LBM_ERR_RUNTIME, "hipEventCreate failed"));
+    s->bytes = bytes;
+    EventRec r;
+    int rc = ev_begin(c, kind, r);
+    if (rc) return give_back(rc);
+    if (what == LBM_SNAPSHOT_MACRO) {
+        rc = (c->cfg.precision == LBM_F64) ? launch_pack_macro<double>(c, static_cast<double*>(s->dev))
+                                           : launch_pack_macro<float>(c, static_cast<float*>(s->dev));
+    } else if (what == LBM_SNAPSHOT_F) {
+        rc = enqueue_view(c, buffer, s->dev, 1);
+    } else {
+        rc = ensure_map(c);
+        const hipError_t e = rc ? hipSuccess : hipMemcpyAsync(s->dev, c->map, bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) rc = fail(LBM_ERR_RUNTIME, std::string("hipMemcpyAsync D2D: ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = ev_end(c, r);
+    if (rc) return give_back(rc);
+    hipError_t e = hipEventRecord(s->packed, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->copy_stream, s->packed, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->host, s->dev, bytes, hipMemcpyDeviceToHost, c->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(s->copied, c->copy_stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        return give_back(fail(LBM_ERR_RUNTIME, std::string("snapshot copy: ") + hipGetErrorString(e)));
+    }
+    *ticket = t;
+    return LBM_OK;
+}
+
+int lbm_snapshot_wait(lbm_ctx* c, int64_t ticket, const void** data, int64_t* bytes) {
+    LBM_CHECK_CTX(c);
+    if (!data || !bytes) return fail(LBM_ERR_INVALID, "null output");
+    SnapSlot* s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c->snap_mu);
+        for (SnapSlot& x : c->snaps)
+            if (x.ticket == ticket && x.state != 0) s = &x;
+    }
+    if (!s) return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
+    LBM_HIP_CHECK(hipEventSynchronize(s->copied), "hipEventSynchronize");
+    {
+        std::lock_guard<std::mutex> lk(c->snap_mu);
+        s->state = 2;
+    }
+    *data = s->host;
+    *bytes = int64_t(s->bytes);
+    return LBM_OK;
+}
+
+int lbm_snapshot_release(lbm_ctx* c, int64_t ticket) {
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(c->snap_mu);
+    for (SnapSlot& x : c->snaps)
+        if (x.ticket == ticket && x.state != 0) {
+            if (x.state == 1) return fail(LBM_ERR_STATE, "snapshot released before lbm_snapshot_wait");
+            x.state = 0;
+            x.ticket = -1;
+            c->snap_cv.notify_all();
+            return LBM_OK;
+        }
+    return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
 }
 
 int lbm_read_state(lbm_ctx* c, int32_t buffer, void* out) {

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -71,6 +73,19 @@ struct EventRec {
     hipEvent_t stop;
 };
 
+// One staging slot of the asynchronous snapshots: device buffer, pinned host
+// buffer, and the event that marks the end of its device-to-host copy.
+struct SnapSlot {
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t cap = 0;        // bytes allocated in both buffers
+    size_t bytes = 0;      // bytes of the current snapshot
+    hipEvent_t copied = nullptr;
+    hipEvent_t packed = nullptr;
+    int64_t ticket = -1;
+    int state = 0;         // 0 free, 1 enqueued, 2 waited (held by the caller)
+};
+
 }  // namespace lbm_host
 
 struct lbm_ctx {
@@ -85,7 +100,8 @@ struct lbm_ctx {
     void* lat[2] = {nullptr, nullptr};  // lat[0] read by odd iterations, lat[1] by even
     void* rho = nullptr;
     void* u = nullptr;
-    int32_t* map = nullptr;
+    int32_t* map = nullptr;      // cell flags, allocated on first read (ensure_map)
+    bool map_valid = false;
     double* partials = nullptr;
     int64_t partials_cap = 0;  // in blocks
     double* diag = nullptr;    // 2 doubles
@@ -94,6 +110,12 @@ struct lbm_ctx {
     double nu = 0, inv_tau = 0, U = 0, scale = 1;
     void* force = nullptr;     // body force, 3 N^3 of the arithmetic type (null: none)
     std::vector<lbm_host::EventRec> events;
+    // asynchronous snapshots (slot table guarded by snap_mu)
+    hipStream_t copy_stream = nullptr;
+    lbm_host::SnapSlot snaps[LBM_SNAPSHOT_SLOTS];
+    int64_t next_ticket = 1;
+    std::mutex snap_mu;
+    std::condition_variable snap_cv;
 };
 
 namespace lbm_host {
@@ -132,6 +154,14 @@ inline void release(lbm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (SnapSlot& s : c->snaps) {
+        if (s.dev) (void)hipFree(s.dev);
+        if (s.host) (void)hipHostFree(s.host);
+        if (s.copied) (void)hipEventDestroy(s.copied);
+        if (s.packed) (void)hipEventDestroy(s.packed);
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     free_events(c);
     for (void* p : {c->lat[0], c->lat[1], c->rho, c->u, (void*)c->map, (void*)c->partials, (void*)c->diag, c->force})
         if (p) (void)hipFree(p);
@@ -284,7 +314,6 @@ inline int launch_initialize(lbm_ctx* c) {
     a.b = c->lat[1];
     a.rho = static_cast<T*>(c->rho);
     a.u = static_cast<T*>(c->u);
-    a.map = c->map;
     a.n3 = c->n3;
     a.stride = c->stride;
     a.n = c->n;
@@ -299,12 +328,13 @@ inline int launch_initialize(lbm_ctx* c) {
 }
 
 template <class T, int KIND>
-inline int launch_view(lbm_ctx* c, const void* p, T* out) {
+inline int launch_view(lbm_ctx* c, const void* p, T* out, int64_t out_stride) {
     lbm::ViewArgs<T> a{};
     a.p = p;
     a.out = out;
     a.n3 = c->n3;
     a.stride = c->stride;
+    a.out_stride = out_stride;
     a.n = c->n;
     a.log2n = c->log2n;
     a.scale = float(c->scale);
@@ -330,6 +360,37 @@ inline int read_device(lbm_ctx* c, int kind, void* dst, const void* src, size_t 
     rc = ev_end(c, r);
     if (rc) return rc;
     LBM_HIP_CHECK(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+// The map buffer, allocated and filled on first use (lbm_read_map, map
+// snapshots); the initialize and compute kernels do not need it.
+inline int ensure_map(lbm_ctx* c) {
+    if (c->map_valid) return LBM_OK;
+    if (!c->map) {
+        const int rc = dev_alloc(reinterpret_cast<void**>(&c->map), size_t(c->n3) * 4, "map");
+        if (rc) return rc;
+    }
+    const int64_t nb = (c->n3 + 255) / 256;
+    hipLaunchKernelGGL(lbm::lbm_map_kernel, dim3(unsigned(nb)), dim3(256), 0, c->stream, c->map, c->n3, c->n, c->log2n);
+    LBM_HIP_CHECK(hipGetLastError(), "map launch");
+    c->map_valid = true;
+    return LBM_OK;
+}
+
+template <class T>
+inline int launch_pack_macro(lbm_ctx* c, T* out) {
+    lbm::PackArgs<T> a{};
+    a.rho = static_cast<const T*>(c->rho);
+    a.u = static_cast<const T*>(c->u);
+    a.out = out;
+    a.n = c->n;
+    a.n3 = c->n3;
+    a.w = c->n - 2;
+    a.w3 = int64_t(a.w) * a.w * a.w;
+    const int64_t nb = (a.w3 + 255) / 256;
+    hipLaunchKernelGGL((lbm::lbm_pack_macro_kernel<T>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
+    LBM_HIP_CHECK(hipGetLastError(), "pack launch");
     return LBM_OK;
 }
 

@@ -723,7 +723,6 @@ template <class T> struct InitArgs {
     void* b;
     T* rho;
     T* u;
-    int32_t* map;
     int64_t n3;
     int64_t stride;
     int32_t n;
@@ -744,7 +743,6 @@ __global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
     const int32_t y = int32_t((c >> a.log2n) & (n - 1));
     const int32_t z = int32_t(c >> (2 * a.log2n));
     const int32_t t = cell_type(x, y, z, n);
-    a.map[c] = t;
     const T qnan = __builtin_nan("");
     const bool st = is_store_macro(t);
     a.rho[c] = st ? T(1) : qnan;
@@ -772,6 +770,17 @@ __global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
     }
 }
 
+// The cell flags (common.h:7-20), materialised only when a caller reads the
+// map: the compute kernels derive them from coordinates in registers.
+static __global__ __launch_bounds__(256) void lbm_map_kernel(int32_t* map, int64_t n3, int32_t n, int32_t log2n) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n3) return;
+    const int32_t x = int32_t(c & (n - 1));
+    const int32_t y = int32_t((c >> log2n) & (n - 1));
+    const int32_t z = int32_t(c >> (2 * log2n));
+    map[c] = cell_type(x, y, z, n);
+}
+
 // ---------------------------------------------------------------------------
 // Push view for lbm_read_f: out[slot(d, i)] = P[slot(d - e_i, i)] decoded,
 // NaN where d - e_i is outside the lattice.
@@ -780,7 +789,8 @@ template <class T> struct ViewArgs {
     const void* p;
     T* out;
     int64_t n3;
-    int64_t stride;
+    int64_t stride;      // CSoA stride of the lattice read
+    int64_t out_stride;  // CSoA stride of the view written (1: cell major)
     int32_t n;
     int32_t log2n;
     float scale, inv_scale;
@@ -806,8 +816,34 @@ __global__ __launch_bounds__(256) void lbm_view_kernel(ViewArgs<T> a) {
             const int64_t s = int64_t(sx) + int64_t(n) * (int64_t(sy) + int64_t(n) * sz);
             v = codec.load(p + slot(s, i, a.stride), w_of<T>(i));
         }
-        a.out[slot(c, i, a.stride)] = v;
+        a.out[slot(c, i, a.out_stride)] = v;
     }
+}
+
+// rho and u of the VTI window x, y, z in [1, N-2] (W = N - 2, x fastest):
+// out[k] = rho, out[W^3 + 3k + d] = u_d, for window point k.
+template <class T> struct PackArgs {
+    const T* rho;
+    const T* u;
+    T* out;
+    int32_t n;
+    int32_t w;
+    int64_t n3;
+    int64_t w3;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void lbm_pack_macro_kernel(PackArgs<T> a) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= a.w3) return;
+    const int64_t w = a.w;
+    const int64_t x = k % w + 1, y = (k / w) % w + 1, z = k / (w * w) + 1;
+    const int64_t c = x + int64_t(a.n) * (y + int64_t(a.n) * z);
+    a.out[k] = a.rho[c];
+    T* v = a.out + a.w3 + 3 * k;
+    v[0] = a.u[c];
+    v[1] = a.u[a.n3 + c];
+    v[2] = a.u[2 * a.n3 + c];
 }
 
 // Final reduction of the per-block (mass, energy) partials, in a fixed order:

@@ -50,7 +50,7 @@ struct lbm_options {
                      "-w  --work_group_size     Specify the work group size \"x,y,z\"          \n"
                      "-s  --stride              Specify the stride used in CSoA memory layout  \n"
                      "-F  --use_double          Make use of \"double\" type                    \n"
-                     "-o  --optimize            Request fast math (kernels stay exact here)    \n"
+                     "-o  --optimize            Request fast math (fused collision arithmetic) \n"
                      "-v  --vtk_path            Specify where store VTI files                  \n"
                      "-p  --dump_path           Specify where store dumps                      \n"
                      "-m  --dump_map            Dump the lattice map                           \n"

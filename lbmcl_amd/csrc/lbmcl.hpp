@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <iostream>
 #include <limits>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <type_traits>
@@ -22,6 +23,7 @@
 
 #include "lbm_hip.h"
 #include "lbm_io.hpp"
+#include "lbm_output.hpp"
 
 #define LBM_CHECK_EXIT(call, what) lbm_check_exit((call), (what), __FILE__, __LINE__)
 
@@ -64,10 +66,8 @@ class LBMCL {
     std::string device_name;
     lbm_ctx* ctx = nullptr;
 
-    std::vector<int32_t> map_values;
-    std::vector<T> f_values;
-    std::vector<T> rho_values;
-    std::vector<T> u_values;
+    // files are written by a background thread from asynchronous snapshots
+    std::unique_ptr<lbm_out::Writer> writer;
 
     size_t f_dim() const { return dim * dim * dim * lbm_io::Q; }
     size_t u_dim() const { return dim * dim * dim * 3; }
@@ -82,7 +82,7 @@ class LBMCL {
 
     size_t device_memory_size_b() const {
         const size_t lattice = (storage == LBM_STORE_NATIVE) ? f_size() : f_dim() * 2;
-        return lattice * 2 + u_size() + rho_size() + map_size();
+        return lattice * 2 + u_size() + rho_size() + (dump_map ? map_size() : 0);  // the map only when dumped
     }
     size_t device_memory_size_k() const { return device_memory_size_b() / (1 << 10); }
     size_t device_memory_size_m() const { return device_memory_size_b() / (1 << 20); }
@@ -122,23 +122,28 @@ class LBMCL {
         return b.str();
     }
 
+    // The dumps: a snapshot is enqueued behind the work enqueued so far and the
+    // background writer formats the file while the device keeps computing.
+    int64_t snapshot(int32_t what, int32_t buffer, const char* name) {
+        int64_t ticket = 0;
+        LBM_CHECK_EXIT(lbm_snapshot(ctx, what, buffer, &ticket), name);
+        return ticket;
+    }
+
     void storeMap() {
-        LBM_CHECK_EXIT(lbm_read_map(ctx, map_values.data()), READ_MAP_NAME);
-        lbm_io::write_map(dump_path + "/map.dump", map_values.data(), dim);
+        const int64_t t = snapshot(LBM_SNAPSHOT_MAP, 0, READ_MAP_NAME);
+        writer->push(lbm_out::Writer::MAP_DUMP, t, dump_path + "/map.dump");
     }
 
     // buffer 0 is the reference's f_collide, 1 its f_stream
     void storeF(int buffer, size_t iteration) {
-        LBM_CHECK_EXIT(lbm_read_f(ctx, buffer, f_values.data()), READ_F_NAME);
-        lbm_io::write_f(lbm_io::numbered_path(dump_path, "f_", iteration, iterations, ".dump"), f_values.data(), dim,
-                        stride);
+        const int64_t t = snapshot(LBM_SNAPSHOT_F, buffer, READ_F_NAME);
+        writer->push(lbm_out::Writer::F_DUMP, t, lbm_io::numbered_path(dump_path, "f_", iteration, iterations, ".dump"));
     }
 
     void storeData(size_t iteration) {
-        LBM_CHECK_EXIT(lbm_read_rho(ctx, rho_values.data()), READ_RHO_NAME);
-        LBM_CHECK_EXIT(lbm_read_u(ctx, u_values.data()), READ_U_NAME);
-        lbm_io::write_vti(lbm_io::numbered_path(vtk_path, "lbmcl.", iteration, iterations, ".vti"),
-                          rho_values.data(), u_values.data(), dim);
+        const int64_t t = snapshot(LBM_SNAPSHOT_MACRO, 0, READ_RHO_NAME);
+        writer->push(lbm_out::Writer::VTI, t, lbm_io::numbered_path(vtk_path, "lbmcl.", iteration, iterations, ".vti"));
     }
 
     int selectDevice(int deviceID) {
@@ -234,16 +239,13 @@ class LBMCL {
         cfg.xcd_remap = 1;
         LBM_CHECK_EXIT(lbm_create(&cfg, &ctx), "lbm_create");
 
-        if (dump_map) map_values.resize(map_dim());
-        if (dump_f) f_values.resize(f_dim());
-        if (dump_data) {
-            rho_values.resize(rho_dim());
-            u_values.resize(u_dim());
-        }
+        if (dump_map || dump_f || dump_data)
+            writer.reset(new lbm_out::Writer(ctx, dim, !std::is_same<T, float>::value, lbm_out::io_threads()));
     }
 
-    // Enqueue the whole simulation (lbmcl.hpp:490-521); the dumps read back
-    // and write files as they come, so this blocks when dumping.
+    // Enqueue the whole simulation (lbmcl.hpp:490-521).  Dumps are snapshots
+    // written in the background; this blocks only when every snapshot slot is
+    // still waiting for its file to be written.
     void performSimulation() {
         LBM_CHECK_EXIT(lbm_initialize(ctx), INITIALIZE_KERNEL_NAME);
         if (dump_map) storeMap();
@@ -257,7 +259,18 @@ class LBMCL {
         }
     }
 
-    void waitCompletion() { LBM_CHECK_EXIT(lbm_finish(ctx), "lbm_finish"); }
+    // The device is idle and every file is written.
+    void waitCompletion() {
+        LBM_CHECK_EXIT(lbm_finish(ctx), "lbm_finish");
+        if (writer) {
+            std::string what;
+            const int rc = writer->drain(&what);
+            if (rc) {
+                std::cerr << __FILE__ << ":" << __LINE__ << " " << what << "(" << rc << ")" << std::endl;
+                std::exit(rc);
+            }
+        }
+    }
 
     void performSimulationAndWait() {
         performSimulation();
@@ -335,6 +348,7 @@ class LBMCL {
     }
 
     ~LBMCL() {
+        writer.reset();  // finishes the pending files; uses ctx
         if (ctx) lbm_destroy(ctx);
     }
 };

@@ -19,6 +19,16 @@ from lbmcl_amd.vti import read_vti
 SELFTEST = native_build.IO_SELFTEST
 
 
+def _c_float(fmt, x):
+    """printf-style formatting as the reference's ostream does it: C prints the
+    sign of a NaN ("-nan"), Python does not."""
+    x = float(x)
+    if math.isnan(x):
+        s = "-nan" if math.copysign(1.0, x) < 0 else "nan"
+        return s if fmt == "%.16e" else f"{s:>8}"
+    return fmt % x
+
+
 def _digits(v):  # lbmcl.hpp:13
     return int(math.log10(v)) + 1 if v > 0 else 1
 
@@ -56,7 +66,7 @@ def _f_dump(f, n, stride):  # lbmcl.hpp:206-258
             out.append(" " * (dd * 3 + 5) + "".join(f"{q:>8} " for q in range(19)) + "\n")
             for x in range(n):
                 c = x + y * n + z * n * n
-                vals = "".join("%8.6f " % float(f[((c // stride) * 19 + q) * stride + (c & (stride - 1))])
+                vals = "".join(_c_float("%8.6f", f[((c // stride) * 19 + q) * stride + (c & (stride - 1))]) + " "
                                for q in range(19))
                 out.append(f"{'(':>{dd}}{x},{y},{z}) " + vals + "\n")
             out.append("\n")
@@ -76,7 +86,7 @@ def _vti(rho, u, n, dtype_name):  # lbmcl.hpp:261-334
            f'        <DataArray type="{dtype_name}" Name="rho" NumberOfComponents="1" format="ascii">\n']
     for z in range(1, n - 1):
         for y in range(1, n - 1):
-            out.append("".join("%.16e " % float(rho[x + y * n + z * n * n]) for x in range(1, n - 1)) + "\n")
+            out.append("".join(_c_float("%.16e", rho[x + y * n + z * n * n]) + " " for x in range(1, n - 1)) + "\n")
     out.append("        </DataArray>\n")
     out.append(f'        <DataArray type="{dtype_name}" Name="v" NumberOfComponents="3" format="ascii">\n')
     for z in range(1, n - 1):
@@ -84,7 +94,7 @@ def _vti(rho, u, n, dtype_name):  # lbmcl.hpp:261-334
             row = []
             for x in range(1, n - 1):
                 c = x + y * n + z * n * n
-                row.append("%.16e %.16e %.16e " % (float(u[c]), float(u[n3 + c]), float(u[2 * n3 + c])))
+                row.append("".join(_c_float("%.16e", u[k * n3 + c]) + " " for k in range(3)))
             out.append("".join(row) + "\n")
     out.append("        </DataArray>\n      </PointData>\n    </Piece>\n  </ImageData>\n</VTKFile>\n")
     return "".join(out)
@@ -97,10 +107,14 @@ def selftest_bin():
     return SELFTEST
 
 
-@pytest.mark.parametrize("n,stride,prec,it,its", [(8, 4, "f32", 7, 120), (8, 512, "f64", 3, 3), (16, 1, "f32", 0, 0),
-                                                  (4, 64, "f64", 12, 1000)])
-def test_dump_formats_byte_exact(selftest_bin, tmp_path, n, stride, prec, it, its):
-    subprocess.run([selftest_bin, str(tmp_path), str(n), str(stride), prec, str(it), str(its)], check=True)
+@pytest.mark.parametrize("n,stride,prec,it,its,threads", [(8, 4, "f32", 7, 120, 3), (8, 512, "f64", 3, 3, 1),
+                                                          (16, 1, "f32", 0, 0, 5), (4, 64, "f64", 12, 1000, 2),
+                                                          (32, 8, "f64", 5, 10, 7)])
+def test_dump_formats_byte_exact(selftest_bin, tmp_path, n, stride, prec, it, its, threads):
+    """Both writers (iostream and the threaded to_chars output path) against an
+    independent statement of the reference writers, byte for byte."""
+    subprocess.run([selftest_bin, str(tmp_path), str(n), str(stride), prec, str(it), str(its), str(threads)],
+                   check=True)
     dt = np.float32 if prec == "f32" else np.float64
     rho = np.fromfile(tmp_path / "rho.bin", dtype=dt)
     u = np.fromfile(tmp_path / "u.bin", dtype=dt)
@@ -109,9 +123,15 @@ def test_dump_formats_byte_exact(selftest_bin, tmp_path, n, stride, prec, it, it
     w = _digits(its)
     vti_path = tmp_path / f"lbmcl.{it:0{w}d}.vti"
     f_path = tmp_path / f"f_{it:0{w}d}.dump"
-    assert (tmp_path / "map.dump").read_text() == _map_dump(m, n)
-    assert f_path.read_text() == _f_dump(f, n, stride)
-    assert vti_path.read_text() == _vti(rho, u, n, "Float32" if prec == "f32" else "Float64")
+    want_map, want_f = _map_dump(m, n), _f_dump(f, n, stride)
+    want_vti = _vti(rho, u, n, "Float32" if prec == "f32" else "Float64")
+    if n >= 8:
+        assert "-nan" in want_vti and "-inf" in want_vti and "-0.0000000000000000e+00" in want_vti
+        assert "-nan" in want_f and "     inf" in want_f
+    for prefix in ("", "fast_"):
+        assert (tmp_path / f"{prefix}map.dump").read_text() == want_map, prefix
+        assert (tmp_path / f"{prefix}f_{it:0{w}d}.dump").read_text() == want_f, prefix
+        assert (tmp_path / f"{prefix}lbmcl.{it:0{w}d}.vti").read_text() == want_vti, prefix
     # and the VTI reads back exactly (values print with 17 significant digits)
     v = read_vti(str(vti_path))
     assert v.header_type == "UInt64" and v.dimensions == (n - 2,) * 3

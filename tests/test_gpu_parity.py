@@ -238,3 +238,19 @@ def test_full_size_512_f32():
         assert nan_equal(rg, ro), mismatch_count(rg, ro)
         ug = g.read_u()
         assert nan_equal(ug, o.u), mismatch_count(ug, o.u)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_cavity_128_1000_steps(precision):
+    """The 128^3 GPU configuration: stride 64, 1000 iterations, rho and u after
+    the last one bit-identical to the oracle at the same precision (and so
+    within 1e-5 / 1e-12 relative)."""
+    n, steps = 128, 1000
+    g, o = _pair(n, 64, precision)
+    with g:
+        g.compute_range(1, steps - 1, 0)
+        g.compute(steps, update_macro=True)
+        for _ in range(steps - 1):
+            o.step(update_macro=False)
+        o.step(update_macro=True)
+        _assert_macro_equal(g, o, n, precision, f"128^3 x {steps} {precision}")
