@@ -43,7 +43,9 @@ def gaussian_random_field(n, correlation=8.0, amplitude=1e-6, seed=0, components
 
 
 def _context(dim, velocity, viscosity, precision, stride, kernel=0):
-    return capi.Context(dim, viscosity=viscosity, velocity=velocity, stride=stride or dim, precision=precision,
+    # default CSoA stride: 4096 (the fastest measured), at most dim^3
+    stride = stride or min(4096, dim ** 3)
+    return capi.Context(dim, viscosity=viscosity, velocity=velocity, stride=stride, precision=precision,
                         profile=False, raw_parameters=True, kernel=kernel)
 
 
