@@ -14,6 +14,8 @@
 // Device memory: M checkpoints + (K - 1) tape states + 2 lattices + 2 adjoint
 // lattices, i.e. about 2 sqrt(T) states for K = sqrt(T).
 // No counterpart in the reference.
+#include <type_traits>
+
 #include "lbm_adjoint.hpp"
 #include "lbm_ctx.hpp"
 
@@ -76,12 +78,25 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         if (rc) return rc;
         device_bytes += double(S);
     }
+    // dJ/dg: FP64 sum; FP32 runs accumulate in an FP32 buffer flushed into it
+    // every ADJ_FLUSH steps (half the read-modify-write traffic per step)
     double* grad_g = nullptr;
+    T* grad_acc = nullptr;
+    constexpr bool staged = std::is_same<T, float>::value;
     if (grad_g_host) {
-        const int rc = buf.alloc(reinterpret_cast<void**>(&grad_g), size_t(3 * c->n3) * sizeof(double), "dJ/dg");
+        const size_t n = size_t(3 * c->n3);
+        int rc = buf.alloc(reinterpret_cast<void**>(&grad_g), n * sizeof(double), "dJ/dg");
         if (rc) return rc;
-        device_bytes += double(3 * c->n3) * sizeof(double);
-        LBM_HIP_CHECK(hipMemsetAsync(grad_g, 0, size_t(3 * c->n3) * sizeof(double), c->stream), "hipMemsetAsync");
+        device_bytes += double(n) * sizeof(double);
+        LBM_HIP_CHECK(hipMemsetAsync(grad_g, 0, n * sizeof(double), c->stream), "hipMemsetAsync");
+        if (staged) {
+            rc = buf.alloc(reinterpret_cast<void**>(&grad_acc), n * sizeof(T), "dJ/dg accumulator");
+            if (rc) return rc;
+            device_bytes += double(n) * sizeof(T);
+            LBM_HIP_CHECK(hipMemsetAsync(grad_acc, 0, n * sizeof(T), c->stream), "hipMemsetAsync");
+        } else {
+            grad_acc = reinterpret_cast<T*>(grad_g);
+        }
     }
     // vector adjoint kernel: one block of N / V threads per interior row
     const bool vec = adjoint_vec_possible<T>(c);
@@ -143,7 +158,7 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
     LBM_HIP_CHECK(hipEventRecord(e1, c->stream), "hipEventRecord");
 
     lbm::AdjArgs<T> a{};
-    a.grad_g = grad_g;
+    a.grad_acc = grad_acc;
     a.partials = partials;
     a.n3 = c->n3;
     a.stride = c->stride;
@@ -198,6 +213,12 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
             rc = adj_step(Pk, lam[cur], lam[1 - cur], 0, (s0 + j == 0) ? 1 : 0);
             if (rc) return rc;
             cur = 1 - cur;
+            if (staged && grad_acc && (s0 + j) % lbm::ADJ_FLUSH == 0) {
+                const int64_t n = 3 * c->n3;
+                hipLaunchKernelGGL(lbm::lbm_adjoint_flush, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream,
+                                   reinterpret_cast<float*>(grad_acc), grad_g, n);
+                LBM_HIP_CHECK(hipGetLastError(), "adjoint flush launch");
+            }
         }
     }
     hipLaunchKernelGGL(lbm::lbm_adjoint_reduce, dim3(1), dim3(1024), 0, c->stream, partials, nblocks, acc);

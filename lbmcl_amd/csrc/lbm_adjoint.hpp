@@ -34,7 +34,8 @@ template <class T> struct AdjArgs {
     const T* P;        // forward state P_k (native storage, pull form)
     const T* Lnext;    // lambda_{k+1}
     T* Lprev;          // lambda_k
-    double* grad_g;    // 3 N^3, accumulated (may be null)
+    T* grad_acc;       // 3 N^3 dJ/dg accumulator of the arithmetic type (may be null); FP32 runs
+                       // flush it into the FP64 sum every ADJ_FLUSH steps
     double* partials;  // 3 doubles per block: dJ/dU, dJ/domega, J
     int64_t n3;
     int64_t stride;
@@ -254,10 +255,10 @@ __global__ __launch_bounds__(256) void lbm_adjoint_step(AdjArgs<T> a) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) L[j] = a.seed ? T(0) : a.Lnext[slot(c, j, a.stride)];
         adjoint_cell<T>(a, t, z, f, L, lam, dU, dW, J, gg);
-        if (a.grad_g && t == FLUID && !a.seed) {
-            a.grad_g[c] += gg[0];
-            a.grad_g[a.n3 + c] += gg[1];
-            a.grad_g[2 * a.n3 + c] += gg[2];
+        if (a.grad_acc && t == FLUID && !a.seed) {
+            a.grad_acc[c] += T(gg[0]);
+            a.grad_acc[a.n3 + c] += T(gg[1]);
+            a.grad_acc[2 * a.n3 + c] += T(gg[2]);
         }
         adjoint_initial_sources<T>(a, t, x, y, z, lam, dU);
         // scatter to the gather positions; own slots whose destination is a
@@ -390,15 +391,15 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) gv[k][j] = gfl ? gg[k] : 0.0;
     }
-    if (a.grad_g && !a.seed) {
-        typedef double DV __attribute__((ext_vector_type(V)));
+    if (a.grad_acc && !a.seed) {
+        typedef T DV __attribute__((ext_vector_type(V)));
         const int64_t c = int64_t(x0) + oc0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            DV* p = reinterpret_cast<DV*>(a.grad_g + int64_t(k) * a.n3 + c);
+            DV* p = reinterpret_cast<DV*>(a.grad_acc + int64_t(k) * a.n3 + c);
             DV v = *p;
 #pragma unroll
-            for (int j = 0; j < V; ++j) v[j] += gv[k][j];
+            for (int j = 0; j < V; ++j) v[j] += T(gv[k][j]);
             *p = v;
         }
     }
@@ -444,6 +445,20 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
         }
     }
     block_accumulate3(dU, dW, J, lds, a.partials + 3 * lb);
+}
+
+// Steps between flushes of an FP32 dJ/dg accumulator into the FP64 sum.  The
+// flushes follow the global step index (k % ADJ_FLUSH == 0), so results do
+// not depend on the checkpoint interval.
+constexpr int ADJ_FLUSH = 32;
+
+// g[i] += s[i], s[i] = 0 for i < n (FP32 accumulator into the FP64 sum)
+static __global__ __launch_bounds__(256) void lbm_adjoint_flush(float* s, double* g, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        g[i] += double(s[i]);
+        s[i] = 0.0f;
+    }
 }
 
 // acc[0..2] += sum of the per-block partials, in a fixed order
