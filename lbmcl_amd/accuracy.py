@@ -7,6 +7,9 @@ native run on the fluid and lid cells:
   rho      max |rho - rho64| / max |rho64|       (relative, rho ~ 1)
   u        max |u - u64| / max |u64|              (relative to the largest speed)
   u_rms    rms |u - u64| / max |u64|
+  *_at     (x, y, z) of the largest difference
+  *_core   the same maxima over cells at least 4 cells from every wall
+  mass     relative difference of the total mass sum(rho)
   vortex   distance (lattice units) between the primary-vortex centres, in the
            mid-plane y = N/2: the centroid of the positive y-vorticity
            w_y = d ux/dz - d uz/dx (the circulation the lid drives), and the
@@ -89,9 +92,25 @@ def compare(ref, test, n):
     du = np.sqrt(np.sum((np.nan_to_num(u1) - np.nan_to_num(u0)) ** 2, axis=0))[m]
     c0, c1 = vorticity_centroid(u0, n), vorticity_centroid(u1, n)
     s0, s1 = min_speed_centre(u0, n), min_speed_centre(u1, n)
+    # locations (arrays are indexed [z, y, x]) and the core away from the walls
+    dr_full = np.where(m, np.abs(rho1 - rho0), 0.0)
+    du_full = np.where(m, np.sqrt(np.sum((np.nan_to_num(u1) - np.nan_to_num(u0)) ** 2, axis=0)), 0.0)
+    core = np.zeros_like(m)
+    core[4:n - 4, 4:n - 4, 4:n - 4] = True
+
+    def at(a):
+        z, y, x = np.unravel_index(int(np.argmax(a)), a.shape)
+        return [int(x), int(y), int(z)]
+
+    rmax = np.max(np.abs(rho0[m]))
     return {
-        "rho_rel_max": float(np.max(np.abs(rho1[m] - rho0[m])) / np.max(np.abs(rho0[m]))),
+        "rho_rel_max": float(np.max(np.abs(rho1[m] - rho0[m])) / rmax),
+        "rho_rel_max_at": at(dr_full),
+        "rho_rel_max_core": float(np.max(dr_full[core]) / rmax),
+        "mass_rel": float((np.sum(rho1[m]) - np.sum(rho0[m])) / np.sum(rho0[m])),
         "u_rel_max": float(np.max(du) / umax),
+        "u_rel_max_at": at(du_full),
+        "u_rel_max_core": float(np.max(du_full[core]) / umax),
         "u_rel_rms": float(np.sqrt(np.mean(du ** 2)) / umax),
         "vortex_centroid": c1,
         "vortex_centroid_shift": float(np.hypot(c1[0] - c0[0], c1[1] - c0[1])),
