@@ -179,11 +179,15 @@ int lbm_compute(lbm_ctx* c, int64_t iteration, int32_t update_macro, int32_t dia
     const void* src = odd ? c->lat[0] : c->lat[1];
     void* dst = odd ? c->lat[1] : c->lat[0];
     EventRec r;
-    int rc = ev_begin(c, LBM_EVENT_COMPUTE, r);
+    int rc = ev_create(c, LBM_EVENT_COMPUTE, r);
     if (rc) return rc;
-    rc = LBM_DISPATCH(c, launch_compute, c, src, dst, update_macro ? 1 : 0, diagnostics ? 1 : 0);
-    if (rc) return rc;
-    rc = ev_end(c, r);
+    rc = LBM_DISPATCH(c, launch_compute, c, src, dst, update_macro ? 1 : 0, diagnostics ? 1 : 0, r.start, r.stop);
+    if (rc) {
+        if (r.start) (void)hipEventDestroy(r.start);
+        if (r.stop) (void)hipEventDestroy(r.stop);
+        return rc;
+    }
+    rc = ev_keep(c, r);
     if (rc) return rc;
     if (diagnostics) {
         EventRec d;
@@ -441,14 +445,21 @@ int lbm_get_constants(lbm_ctx* c, double* out) {
 
 int lbm_event_count(lbm_ctx* c, int64_t* count) {
     if (!c || !count) return fail(LBM_ERR_INVALID, "null argument");
-    *count = int64_t(c->events.size());
+    *count = int64_t(c->events.done.size() + c->events.pending.size());
     return LBM_OK;
 }
 
 int lbm_event_get(lbm_ctx* c, int64_t index, int32_t* kind, double* ms) {
     LBM_CHECK_CTX(c);
-    if (index < 0 || index >= int64_t(c->events.size())) return fail(LBM_ERR_INVALID, "event index out of range");
-    const EventRec& e = c->events[size_t(index)];
+    const EventLog& L = c->events;
+    const int64_t nd = int64_t(L.done.size());
+    if (index < 0 || index >= nd + int64_t(L.pending.size())) return fail(LBM_ERR_INVALID, "event index out of range");
+    if (index < nd) {
+        if (kind) *kind = L.done[size_t(index)].first;
+        if (ms) *ms = double(L.done[size_t(index)].second);
+        return LBM_OK;
+    }
+    const EventRec& e = L.pending[size_t(index - nd)];
     LBM_HIP_CHECK(hipEventSynchronize(e.stop), "hipEventSynchronize");
     float t = 0.f;
     LBM_HIP_CHECK(hipEventElapsedTime(&t, e.start, e.stop), "hipEventElapsedTime");
@@ -461,10 +472,12 @@ int lbm_total_time_ms(lbm_ctx* c, double* ms) {
     LBM_CHECK_CTX(c);
     if (!ms) return fail(LBM_ERR_INVALID, "null output");
     *ms = 0.0;
-    if (c->events.empty()) return LBM_OK;
-    LBM_HIP_CHECK(hipEventSynchronize(c->events.back().stop), "hipEventSynchronize");
+    const EventLog& L = c->events;
+    if (L.pending.empty()) return LBM_OK;  // the newest pair is never recycled
+    hipEvent_t start = L.first ? L.first : L.pending.front().start;
+    LBM_HIP_CHECK(hipEventSynchronize(L.pending.back().stop), "hipEventSynchronize");
     float t = 0.f;
-    LBM_HIP_CHECK(hipEventElapsedTime(&t, c->events.front().start, c->events.back().stop), "hipEventElapsedTime");
+    LBM_HIP_CHECK(hipEventElapsedTime(&t, start, L.pending.back().stop), "hipEventElapsedTime");
     *ms = double(t);
     return LBM_OK;
 }
@@ -473,7 +486,9 @@ int lbm_kernels_time_ms(lbm_ctx* c, double* ms) {
     LBM_CHECK_CTX(c);
     if (!ms) return fail(LBM_ERR_INVALID, "null output");
     double total = 0.0;
-    for (const EventRec& e : c->events) {
+    for (const auto& d : c->events.done)
+        if (d.first == LBM_EVENT_COMPUTE) total += double(d.second);
+    for (const EventRec& e : c->events.pending) {
         if (e.kind != LBM_EVENT_COMPUTE) continue;
         LBM_HIP_CHECK(hipEventSynchronize(e.stop), "hipEventSynchronize");
         float t = 0.f;

@@ -3,6 +3,7 @@
 // reporting, allocation, kernel selection and the kernel launchers.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,7 +13,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <new>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -73,6 +76,18 @@ struct EventRec {
     hipEvent_t stop;
 };
 
+// Event timing with recycled events.  Pairs are kept in enqueue order; once a
+// pair is complete its duration is read and its events go back to the pool
+// (creating two events per launch would otherwise dominate the host's enqueue
+// cost of small lattices).  The newest pair and the first start event are
+// always kept: the total span needs them.
+struct EventLog {
+    std::vector<hipEvent_t> pool;
+    std::deque<EventRec> pending;                  // not yet read, in enqueue order
+    std::vector<std::pair<int32_t, float>> done;   // (kind, ms) of the read pairs, in order
+    hipEvent_t first = nullptr;                    // start of the first pair, once it was read
+};
+
 // One staging slot of the asynchronous snapshots: device buffer, pinned host
 // buffer, and the event that marks the end of its device-to-host copy.
 struct SnapSlot {
@@ -109,7 +124,7 @@ struct lbm_ctx {
     bool initialized = false;
     double nu = 0, inv_tau = 0, U = 0, scale = 1;
     void* force = nullptr;     // body force, 3 N^3 of the arithmetic type (null: none)
-    std::vector<lbm_host::EventRec> events;
+    lbm_host::EventLog events;
     // asynchronous snapshots (slot table guarded by snap_mu)
     hipStream_t copy_stream = nullptr;
     lbm_host::SnapSlot snaps[LBM_SNAPSHOT_SLOTS];
@@ -125,29 +140,82 @@ inline int set_device(lbm_ctx* c) {
     return LBM_OK;
 }
 
-inline int ev_begin(lbm_ctx* c, int kind, EventRec& r) {
+inline int ev_acquire(lbm_ctx* c, hipEvent_t* e) {
+    if (!c->events.pool.empty()) {
+        *e = c->events.pool.back();
+        c->events.pool.pop_back();
+        return LBM_OK;
+    }
+    LBM_HIP_CHECK(hipEventCreate(e), "hipEventCreate");
+    return LBM_OK;
+}
+
+// An event pair for a launch that stamps it itself (launch_on).
+inline int ev_create(lbm_ctx* c, int kind, EventRec& r) {
     r.kind = kind;
     r.start = r.stop = nullptr;
     if (!c->cfg.profile) return LBM_OK;
-    LBM_HIP_CHECK(hipEventCreate(&r.start), "hipEventCreate");
-    LBM_HIP_CHECK(hipEventCreate(&r.stop), "hipEventCreate");
+    int rc = ev_acquire(c, &r.start);
+    if (!rc) rc = ev_acquire(c, &r.stop);
+    return rc;
+}
+
+// An event pair around enqueued work: start recorded now, stop by ev_end.
+inline int ev_begin(lbm_ctx* c, int kind, EventRec& r) {
+    int rc = ev_create(c, kind, r);
+    if (rc || !c->cfg.profile) return rc;
     LBM_HIP_CHECK(hipEventRecord(r.start, c->stream), "hipEventRecord");
     return LBM_OK;
+}
+
+// Read the durations of the oldest complete pairs and recycle their events.
+inline int ev_harvest(lbm_ctx* c) {
+    EventLog& L = c->events;
+    while (L.pending.size() > 1) {
+        const EventRec& e = L.pending.front();
+        const hipError_t q = hipEventQuery(e.stop);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            break;
+        }
+        if (q != hipSuccess) return fail(LBM_ERR_RUNTIME, std::string("hipEventQuery: ") + hipGetErrorString(q));
+        float t = 0.f;
+        LBM_HIP_CHECK(hipEventElapsedTime(&t, e.start, e.stop), "hipEventElapsedTime");
+        L.done.emplace_back(e.kind, t);
+        if (!L.first && L.done.size() == 1)
+            L.first = e.start;
+        else
+            L.pool.push_back(e.start);
+        L.pool.push_back(e.stop);
+        L.pending.pop_front();
+    }
+    return LBM_OK;
+}
+
+inline int ev_keep(lbm_ctx* c, const EventRec& r) {
+    if (!c->cfg.profile) return LBM_OK;
+    c->events.pending.push_back(r);
+    return c->events.pending.size() > 64 ? ev_harvest(c) : LBM_OK;
 }
 
 inline int ev_end(lbm_ctx* c, EventRec& r) {
     if (!c->cfg.profile) return LBM_OK;
     LBM_HIP_CHECK(hipEventRecord(r.stop, c->stream), "hipEventRecord");
-    c->events.push_back(r);
-    return LBM_OK;
+    return ev_keep(c, r);
 }
 
 inline void free_events(lbm_ctx* c) {
-    for (auto& e : c->events) {
+    EventLog& L = c->events;
+    for (auto& e : L.pending) {
         if (e.start) (void)hipEventDestroy(e.start);
         if (e.stop) (void)hipEventDestroy(e.stop);
     }
-    c->events.clear();
+    for (hipEvent_t e : L.pool) (void)hipEventDestroy(e);
+    if (L.first) (void)hipEventDestroy(L.first);
+    L.pending.clear();
+    L.pool.clear();
+    L.done.clear();
+    L.first = nullptr;
 }
 
 inline void release(lbm_ctx* c) {
@@ -245,8 +313,20 @@ inline int64_t compute_blocks(const lbm_ctx* c) {
 // storage when fast math (-o) is requested.
 inline bool use_fast(const lbm_ctx* c) { return c->cfg.storage != LBM_STORE_NATIVE || c->cfg.optimize != 0; }
 
+// Launch on the context's stream; with events, through hipExtLaunchKernel,
+// which stamps the kernel's own start and end into them (no separate marker
+// packets between launches, like the reference's profiling-queue events).
+template <class K, class A>
+inline void launch_on(lbm_ctx* c, K kernel, dim3 grid, dim3 block, hipEvent_t start, hipEvent_t stop, const A& a) {
+    if (start)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, start, stop, 0, a);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, a);
+}
+
 template <class T, int KIND, bool FAST>
-inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
+inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+                               hipEvent_t e1) {
     lbm::StepArgs<T> a{};
     a.src = src;
     a.dst = dst;
@@ -279,31 +359,30 @@ inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int updat
         const int64_t nb = compute_blocks(c);
         const unsigned bx = unsigned(row_block(c));
         if (kk == K_VEC)
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 16, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
-                               a);
+            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
         else if (kk == K_VEC8)
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_vec<T, KIND, 8, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream,
-                               a);
+            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
         else
-            hipLaunchKernelGGL((lbm::lbm_collide_stream_row<T, KIND, FAST>), dim3(unsigned(nb)), dim3(bx), 0, c->stream, a);
+            launch_on(c, lbm::lbm_collide_stream_row<T, KIND, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
     } else {
         const int b = generic_block(c);
         const int64_t nb = (c->n3 + b - 1) / b;
         a.gx = 0;
-        hipLaunchKernelGGL((lbm::lbm_collide_stream_generic<T, KIND, FAST>), dim3(unsigned(nb)), dim3(b), 0, c->stream,
-                           a);
+        launch_on(c, lbm::lbm_collide_stream_generic<T, KIND, FAST>, dim3(unsigned(nb)), dim3(b), e0, e1, a);
     }
     LBM_HIP_CHECK(hipGetLastError(), "compute launch");
     return LBM_OK;
 }
 
+// e0/e1: events stamped with the kernel's start and end (null: none)
 template <class T, int KIND>
-inline int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag) {
+inline int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0 = nullptr,
+                          hipEvent_t e1 = nullptr) {
     if constexpr (KIND != lbm::ST_NATIVE) {
-        return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
+        return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag, e0, e1);
     } else {
-        if (use_fast(c)) return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag);
-        return launch_compute_impl<T, KIND, false>(c, src, dst, update_macro, diag);
+        if (use_fast(c)) return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag, e0, e1);
+        return launch_compute_impl<T, KIND, false>(c, src, dst, update_macro, diag, e0, e1);
     }
 }
 

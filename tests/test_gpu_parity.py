@@ -208,6 +208,29 @@ def test_events_and_timing():
         assert abs(kt - sum(ms for name, ms in ev if name == "compute")) < 1e-9
 
 
+def test_event_log_over_many_launches():
+    """Hundreds of launches: completed event pairs are read and their events
+    reused; counts, order, per-event times and the totals stay consistent."""
+    n, steps = 64, 700
+    with capi.Context(n, stride=64, profile=True) as g:
+        g.initialize()
+        g.compute_range(1, steps, every=100)
+        g.read_u()
+        g.compute_range(steps + 1, 3, every=0)
+        ev = g.events()
+        names = [e[0] for e in ev]
+        assert len(ev) == 1 + steps + 1 + 3
+        assert names[0] == "initialize" and names[steps + 1] == "read_u" and names.count("compute") == steps + 3
+        assert all(ms > 0 for _, ms in ev)
+        kt, tt = g.kernels_time_ms(), g.total_time_ms()
+        assert abs(kt - sum(ms for name, ms in ev if name == "compute")) < 1e-6 * kt
+        assert sum(ms for _, ms in ev) <= tt * 1.0001
+        g.clear_events()
+        assert g.events() == [] and g.total_time_ms() == 0.0
+        g.compute(steps + 4)
+        assert len(g.events()) == 1 and g.kernels_time_ms() > 0
+
+
 def test_state_roundtrip():
     """lbm_write_state restores what lbm_read_state returned."""
     with capi.Context(32, stride=32) as g:
