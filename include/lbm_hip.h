@@ -64,6 +64,11 @@ extern "C" {
 #define LBM_KERNEL_ROW 2
 #define LBM_KERNEL_VEC 3
 #define LBM_KERNEL_VEC8 4
+/* the tile kernel for small strides (e.g. AoS): tiles of rows staged through
+ * LDS with 16-byte accesses; needs stride <= its x extent (16 cells in FP32,
+ * 8 in FP64, 32 with 16-bit storage) and dim >= max(that extent, 8).
+ * LBM_KERNEL_AUTO selects it when stride * stored bytes <= 32. */
+#define LBM_KERNEL_TILE 5
 
 /* event kinds, named like the reference's events (lbmcl.hpp:26-31) */
 #define LBM_EVENT_INITIALIZE 0

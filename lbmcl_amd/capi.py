@@ -23,7 +23,7 @@ LBM_ERR_STATE = -5
 
 LBM_F32, LBM_F64 = 0, 1
 LBM_STORE_NATIVE, LBM_STORE_FP16_SHIFT, LBM_STORE_BF16_SHIFT = 0, 1, 2
-LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_VEC8 = 0, 1, 2, 3, 4
+LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_VEC8, LBM_KERNEL_TILE = 0, 1, 2, 3, 4, 5
 
 EVENT_NAMES = {0: "initialize", 1: "compute", 2: "read_map", 3: "read_f", 4: "read_rho", 5: "read_u",
                6: "diagnostics"}

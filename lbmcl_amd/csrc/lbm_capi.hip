@@ -73,8 +73,17 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
         return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
     if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
-        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8)
+        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
         return fail(LBM_ERR_INVALID, "unknown kernel selection");
+    if (k.kernel == LBM_KERNEL_TILE) {
+        int64_t d[3];
+        if (k.storage != LBM_STORE_NATIVE) tile_dims<uint16_t>(d);
+        else if (k.precision == LBM_F64) tile_dims<double>(d);
+        else tile_dims<float>(d);
+        if (k.stride > d[0] || k.dim < d[0] || k.dim < d[1] || k.dim < d[2])
+            return fail(LBM_ERR_INVALID, "the tile kernel needs stride <= its x extent (" + std::to_string(d[0]) +
+                                             " cells) and dim >= that extent");
+    }
     if (k.kernel == LBM_KERNEL_ROW && k.dim < 64) return fail(LBM_ERR_INVALID, "the row kernel needs dim >= 64");
     if (k.kernel == LBM_KERNEL_VEC || k.kernel == LBM_KERNEL_VEC8) {
         const int64_t lb = (k.kernel == LBM_KERNEL_VEC) ? 16 : 8;

@@ -9,10 +9,14 @@
 // the reference's stale initial value (kernels.cl:293-317, 421).  The gathered
 // populations, and hence rho, u and every later state, equal the reference's.
 //
-// One thread per cell.  HBM-bound (152 B/cell/step in FP32, 304 in FP64, 76
-// with 16-bit shifted storage); no MFMA -- nothing here is a contraction.
-// Everything is compiled without floating-point contraction, so each value is
-// the reference's expression evaluated operation by operation.
+// Three forms of the step with identical results: the vector row kernel (each
+// lane owns 8 or 16 bytes of every population, i.e. 1-8 consecutive cells),
+// the row kernel and the generic kernel (one thread per cell).  HBM-bound (152
+// B/cell/step in FP32, 304 in FP64, 76 with 16-bit shifted storage); no MFMA
+// -- nothing here is a contraction.  Everything is compiled without
+// floating-point contraction, so each value is the reference's expression
+// evaluated operation by operation; the fused collision (-o, and 16-bit
+// storage) is a different expression that uses fma explicitly.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -657,6 +661,198 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         __shared__ double lds[2 * 16];
         block_reduce2(m, e, lds);
         if (threadIdx.x == 0) {
+            a.partials[2 * lb] = m;
+            a.partials[2 * lb + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel for small CSoA strides: a stride block of one population of at
+// most 32 bytes (S = 1 is AoS).  There the populations of neighbouring cells
+// share cache lines, so per-population accesses fetch each line many times
+// from L2.  A block takes TY x TZ rows and B cells along x (B a multiple of
+// the stride): it loads the B-cell span of each of the (TY + 2)(TZ + 2) source
+// rows once -- 19 B contiguous stored values, since the span starts a stride
+// block -- with 16-byte loads into LDS, plus the x halo cells x0 - 1 and
+// x0 + B for the five populations that cross each side, gathers every cell's
+// populations from LDS, and writes each destination span back through LDS
+// with 16-byte stores.  Wall rows inside a tile are neither gathered nor
+// written; the row's wall cells get their initial pull-form values back.
+// ---------------------------------------------------------------------------
+// Tile shapes (B cells along x, TY x TZ rows) per stored size, measured on
+// MI355X at 512^3 (FP32, 16-bit) and 256^3 (FP64) over strides 1-16: thin x
+// extents with tall y-z tiles (60 source rows for 32 destination rows) and two
+// blocks per CU beat the 128-byte-wide tiles by 1.5x.
+#ifndef LBM_TILE_4B
+#define LBM_TILE_4B 16, 4, 8
+#endif
+#ifndef LBM_TILE_8B
+#define LBM_TILE_8B 8, 4, 8
+#endif
+#ifndef LBM_TILE_2B
+#define LBM_TILE_2B 32, 4, 4
+#endif
+template <int B_, int TY_, int TZ_> struct TileDims {
+    static constexpr int B = B_, TY = TY_, TZ = TZ_;
+};
+template <class S> struct TileShape;
+template <> struct TileShape<float> : TileDims<LBM_TILE_4B> {};
+template <> struct TileShape<double> : TileDims<LBM_TILE_8B> {};
+template <> struct TileShape<_Float16> : TileDims<LBM_TILE_2B> {};
+template <> struct TileShape<uint16_t> : TileDims<LBM_TILE_2B> {};
+
+// populations with e_x = +1 and e_x = -1, in index order
+__device__ __forceinline__ constexpr int xp_slot(int i) {
+    return i == 1 ? 0 : i == 7 ? 1 : i == 10 ? 2 : i == 11 ? 3 : i == 15 ? 4 : -1;
+}
+__device__ __forceinline__ constexpr int xm_slot(int i) {
+    return i == 3 ? 0 : i == 8 ? 1 : i == 9 ? 2 : i == 13 ? 3 : i == 17 ? 4 : -1;
+}
+
+template <class T, int KIND> struct TileThreads {
+    using TS = TileShape<typename Codec<T, KIND>::S>;
+    static constexpr int value = TS::B * TS::TY * TS::TZ;
+};
+
+template <class T, int KIND, bool FAST>
+__global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_stream_tile(StepArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    using TS = TileShape<S>;
+    constexpr int B = TS::B, TY = TS::TY, TZ = TS::TZ;
+    constexpr int NT = B * TY * TZ;
+    constexpr int SY = TY + 2, NR = (TY + 2) * (TZ + 2);  // source rows of the tile
+    constexpr int SPAN = Q * B;                            // stored values of one row span
+    constexpr int VE = 16 / int(sizeof(S));                // values per 16-byte access
+    constexpr int CH = SPAN / VE;                          // 16-byte chunks per span
+    static_assert(SPAN % VE == 0, "span must be whole 16-byte chunks");
+    constexpr int PER = (NR * CH + NT - 1) / NT;           // chunks per thread
+    __shared__ __align__(16) S span[NR][SPAN];
+    __shared__ S halo_l[NR][5];  // cell x0 - 1: populations with e_x = +1
+    __shared__ S halo_r[NR][5];  // cell x0 + B: populations with e_x = -1
+    __shared__ double lds_red[2 * (NT / 64)];
+
+    const C codec{a.scale, a.inv_scale};
+    const int32_t n = a.n;
+    const int32_t x0 = int32_t(blockIdx.x) * B, y0 = int32_t(blockIdx.y) * TY, z0 = int32_t(blockIdx.z) * TZ;
+    const int tid = int(threadIdx.x);
+    const S* __restrict__ src = static_cast<const S*>(a.src);
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+    const int ls = __builtin_ctzll(uint64_t(a.stride));  // log2 stride (<= log2 B)
+    const int smask = (1 << ls) - 1;
+    auto row_first = [&](int32_t yy, int32_t zz) -> int64_t {  // slot of (x0, yy, zz), population 0
+        return 19 * (int64_t(x0) + int64_t(n) * (int64_t(yy) + int64_t(n) * zz));
+    };
+    auto csoa = [&](int xs, int i) -> int {  // position of (cell x0 + xs, population i) in a span
+        return (((xs >> ls) * Q) << ls) + (i << ls) + (xs & smask);
+    };
+
+    // 1. source spans: all loads first, then the LDS writes
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    U4 buf[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q = tid + k * NT;
+        const int r = q / CH, ch = q - (q / CH) * CH;
+        const int32_t yy = y0 - 1 + r % SY, zz = z0 - 1 + r / SY;
+        buf[k] = U4{0u, 0u, 0u, 0u};
+        if (q < NR * CH && yy >= 0 && yy < n && zz >= 0 && zz < n)
+            buf[k] = *reinterpret_cast<const U4*>(src + row_first(yy, zz) + ch * VE);
+    }
+    // x halos: 5 populations per side and source row
+    constexpr int NH = NR * 10, HPER = (NH + NT - 1) / NT;
+    S hv[HPER];
+#pragma unroll
+    for (int k = 0; k < HPER; ++k) {
+        const int hq = tid + k * NT;
+        hv[k] = S(0);
+        const int r = hq / 10, side = (hq / 5) % 2, j = hq % 5;
+        const int32_t yy = y0 - 1 + r % SY, zz = z0 - 1 + r / SY;
+        const int32_t xx = side == 0 ? x0 - 1 : x0 + B;
+        const int i = side == 0 ? (j == 0 ? 1 : j == 1 ? 7 : j == 2 ? 10 : j == 3 ? 11 : 15)
+                                : (j == 0 ? 3 : j == 1 ? 8 : j == 2 ? 9 : j == 3 ? 13 : 17);
+        if (hq < NH && yy >= 0 && yy < n && zz >= 0 && zz < n && xx >= 0 && xx < n)
+            hv[k] = src[slot(int64_t(xx) + int64_t(n) * (int64_t(yy) + int64_t(n) * zz), i, a.stride)];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q = tid + k * NT;
+        if (q < NR * CH) {
+            const int r = q / CH, ch = q - (q / CH) * CH;
+            *reinterpret_cast<U4*>(&span[r][ch * VE]) = buf[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < HPER; ++k) {
+        const int hq = tid + k * NT;
+        if (hq < NH) {
+            const int r = hq / 10, side = (hq / 5) % 2, j = hq % 5;
+            if (side == 0) halo_l[r][j] = hv[k];
+            else halo_r[r][j] = hv[k];
+        }
+    }
+    __syncthreads();
+
+    // 2. one cell per thread
+    const int xl = tid % B, ty = (tid / B) % TY, tz = tid / (B * TY);
+    const int32_t x = x0 + xl, y = y0 + ty, z = z0 + tz;
+    const bool row_ok = y >= 1 && y <= n - 2 && z >= 1 && z <= n - 2;
+    const int32_t t = finish_type(x_flags(x, n) | row_flags(y, z, n));
+    T f[Q];
+    T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
+    if (row_ok && t != WALL) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int r = (ty + 1 - ey_of(i)) + SY * (tz + 1 - ez_of(i));
+            const int xs = xl - ex_of(i);
+            S v;
+            if (ex_of(i) == 1 && xs < 0) v = halo_l[r][xp_slot(i)];
+            else if (ex_of(i) == -1 && xs >= B) v = halo_r[r][xm_slot(i)];
+            else v = span[r][csoa(xs, i)];
+            f[i] = codec.decode(v, w_of<T>(i));
+        }
+        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        const int64_t c = int64_t(x) + int64_t(n) * (int64_t(y) + int64_t(n) * z);
+        if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+        if (a.update_macro && is_store_macro(t)) {
+            a.rho[c] = rho;
+            a.u[c] = ux;
+            a.u[a.n3 + c] = uy;
+            a.u[2 * a.n3 + c] = uz;
+        }
+    } else if (row_ok) {  // the row's wall cells keep their initial values
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = wall_p0<T>(i, x == 0 ? -1 : +1, y, z, n, a.lid_eq);
+    }
+    __syncthreads();  // every gather is done: the spans are reused for the output
+
+    // 3. outputs through LDS (destination row (ty, tz) uses span row ty + TY tz)
+    const int ro = ty + TY * tz;
+    if (row_ok) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) span[ro][csoa(xl, i)] = codec.encode(f[i], w_of<T>(i));
+    }
+    __syncthreads();
+    constexpr int PERO = (TY * TZ * CH + NT - 1) / NT;
+#pragma unroll
+    for (int k = 0; k < PERO; ++k) {
+        const int q = tid + k * NT;
+        if (q < TY * TZ * CH) {
+            const int r = q / CH, ch = q - (q / CH) * CH;
+            const int32_t yy = y0 + r % TY, zz = z0 + r / TY;
+            if (yy >= 1 && yy <= n - 2 && zz >= 1 && zz <= n - 2)
+                __builtin_nontemporal_store(*reinterpret_cast<const U4*>(&span[r][ch * VE]),
+                                            reinterpret_cast<U4*>(dst + row_first(yy, zz) + ch * VE));
+        }
+    }
+
+    if (a.diagnostics) {
+        double m = 0.0, e = 0.0;
+        if (row_ok && t != WALL && is_store_macro(t)) diag_terms<T>(rho, ux, uy, uz, m, e);
+        block_reduce2(m, e, lds_red);
+        if (threadIdx.x == 0) {
+            const uint32_t lb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
             a.partials[2 * lb] = m;
             a.partials[2 * lb + 1] = e;
         }

@@ -53,6 +53,8 @@ def test_library_exports_every_declared_symbol():
     (dict(dim=8, precision="f64", storage="fp16"), capi.LBM_ERR_INVALID, "FP32"),
     (dict(dim=32, kernel=capi.LBM_KERNEL_ROW), capi.LBM_ERR_INVALID, "row kernel"),
     (dict(dim=128, stride=2, kernel=capi.LBM_KERNEL_VEC), capi.LBM_ERR_INVALID, "vector kernel"),
+    (dict(dim=128, stride=64, kernel=capi.LBM_KERNEL_TILE), capi.LBM_ERR_INVALID, "tile kernel"),
+    (dict(dim=8, stride=1, kernel=capi.LBM_KERNEL_TILE), capi.LBM_ERR_INVALID, "tile kernel"),
     (dict(dim=8, viscosity=-1.0), capi.LBM_ERR_INVALID, "viscosity"),
     (dict(dim=8, velocity=float("nan")), capi.LBM_ERR_INVALID, "velocity"),
     (dict(dim=8, lws=(64, 64, 1)), capi.LBM_ERR_INVALID, "work-group"),

@@ -69,6 +69,14 @@ CASES = [
     (128, 128, "f32", capi.LBM_KERNEL_VEC8, 6),
     (64, 64 ** 3, "f64", capi.LBM_KERNEL_VEC8, 8),
     (64, 1, "f64", capi.LBM_KERNEL_VEC8, 8),
+    # the tile kernel (small strides, LDS-staged spans)
+    (32, 1, "f32", capi.LBM_KERNEL_TILE, 12),
+    (32, 16, "f32", capi.LBM_KERNEL_TILE, 12),
+    (128, 1, "f32", capi.LBM_KERNEL_TILE, 6),
+    (128, 8, "f32", capi.LBM_KERNEL_TILE, 6),
+    (16, 1, "f64", capi.LBM_KERNEL_TILE, 12),
+    (64, 2, "f64", capi.LBM_KERNEL_TILE, 8),
+    (128, 8, "f64", capi.LBM_KERNEL_TILE, 6),
 ]
 
 
@@ -100,6 +108,8 @@ FAST_CASES = [
     (128, 64, "f32", capi.LBM_KERNEL_VEC8, 8),
     (256, 256, "f32", capi.LBM_KERNEL_VEC, 5),
     (128, 128, "f64", capi.LBM_KERNEL_VEC, 6),
+    (64, 2, "f32", capi.LBM_KERNEL_TILE, 8),
+    (32, 4, "f64", capi.LBM_KERNEL_TILE, 8),
 ]
 
 
@@ -118,11 +128,12 @@ def test_fused_collision_matches_oracle(n, stride, precision, kernel, steps):
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-@pytest.mark.parametrize("n,kernel,steps", [(16, capi.LBM_KERNEL_GENERIC, 12), (64, capi.LBM_KERNEL_ROW, 12),
-                                            (256, capi.LBM_KERNEL_VEC8, 4), (512, capi.LBM_KERNEL_VEC, 2)])
-def test_shifted_storage_matches_oracle(storage, n, kernel, steps):
-    label = f"{storage} n={n} k={kernel}"
-    g, o = _pair(n, n, "f32", storage=storage, kernel=kernel)
+@pytest.mark.parametrize("n,stride,kernel,steps", [(16, 16, capi.LBM_KERNEL_GENERIC, 12), (64, 64, capi.LBM_KERNEL_ROW, 12),
+                                                   (256, 256, capi.LBM_KERNEL_VEC8, 4), (512, 512, capi.LBM_KERNEL_VEC, 2),
+                                                   (64, 1, capi.LBM_KERNEL_TILE, 8), (128, 32, capi.LBM_KERNEL_TILE, 4)])
+def test_shifted_storage_matches_oracle(storage, n, stride, kernel, steps):
+    label = f"{storage} n={n} s={stride} k={kernel}"
+    g, o = _pair(n, stride, "f32", storage=storage, kernel=kernel)
     with g:
         for it in range(1, steps + 1):
             g.compute(it, update_macro=True)
@@ -135,11 +146,15 @@ def test_shifted_storage_matches_oracle(storage, n, kernel, steps):
             assert nan_equal(fg, fo), f"{label}: f view {buf} differs in {mismatch_count(fg, fo)} slots"
 
 
-@pytest.mark.parametrize("n,precision,kernel", [(16, "f32", capi.LBM_KERNEL_GENERIC), (64, "f64", capi.LBM_KERNEL_ROW),
-                                                (128, "f32", capi.LBM_KERNEL_ROW), (256, "f32", capi.LBM_KERNEL_VEC),
-                                                (128, "f64", capi.LBM_KERNEL_VEC)])
-def test_diagnostics(n, precision, kernel):
-    g, o = _pair(n, n, precision, kernel=kernel)
+@pytest.mark.parametrize("n,stride,precision,kernel", [(16, 16, "f32", capi.LBM_KERNEL_GENERIC),
+                                                       (64, 64, "f64", capi.LBM_KERNEL_ROW),
+                                                       (128, 128, "f32", capi.LBM_KERNEL_ROW),
+                                                       (256, 256, "f32", capi.LBM_KERNEL_VEC),
+                                                       (128, 128, "f64", capi.LBM_KERNEL_VEC),
+                                                       (64, 1, "f32", capi.LBM_KERNEL_TILE),
+                                                       (32, 2, "f64", capi.LBM_KERNEL_TILE)])
+def test_diagnostics(n, stride, precision, kernel):
+    g, o = _pair(n, stride, precision, kernel=kernel)
     with g:
         for it in range(1, 7):
             g.compute(it, update_macro=True, diagnostics=(it % 3 == 0))
