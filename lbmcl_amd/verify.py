@@ -7,7 +7,9 @@ the digit count of `iterations`), replaces NaN by 0 and prints the mean squared
 error and the maximum absolute error of rho and v.  The reference has no
 pass/fail threshold; the optional ``--max-*`` options add one (exit status 1 if
 any iteration exceeds it).  Reads files with the numpy-only reader in
-lbmcl_amd.vti instead of pyvista.
+lbmcl_amd.vti instead of pyvista; ``--pyvista`` additionally reads every file
+with pyvista, when it is importable, and requires both readers to return the
+same arrays.
 """
 from __future__ import annotations
 
@@ -35,15 +37,42 @@ def metrics(target_rho, target_v, pred_rho, pred_v):
             float(np.max(np.abs(tr - pr))), float(np.max(np.abs(tv - pv))))
 
 
-def verify(iterations, every, target_path, prediction_path, out=sys.stdout):
+def pyvista_reader():
+    """pyvista's VTI reader (verify.py:44-45) as a function path -> {name: array}, or None."""
+    try:
+        import pyvista
+    except ImportError:
+        return None
+
+    def read(path):
+        mesh = pyvista.read(path)
+        return {name: np.asarray(mesh.point_data[name]) for name in ("rho", "v")}
+
+    return read
+
+
+def cross_check(path, ours, theirs):
+    """Both readers must agree exactly (NaN where NaN)."""
+    for name in ("rho", "v"):
+        a = np.asarray(ours[name], dtype=np.float64).reshape(-1)
+        b = np.asarray(theirs[name], dtype=np.float64).reshape(-1)
+        if a.shape != b.shape or not np.array_equal(a, b, equal_nan=True):
+            raise ValueError(f"{path}: {name} differs between lbmcl_amd.vti and pyvista")
+
+
+def verify(iterations, every, target_path, prediction_path, out=sys.stdout, other_reader=None):
     """Return a list of (iteration, mse_rho, mse_v, mae_rho, mae_v)."""
     w = width_of(iterations)
     print("{0:^{w}}  {1:^13}  {2:^13}  {3:^13}  {4:^13}".format("#it", "MSE_RHO", "MSE_U", "MAE_RHO", "MAE_U", w=w),
           file=out)
     rows = []
     for i in range(0, iterations + 1, every):
-        t = read_vti("{p}/ldc.0.{i:0{w}}.vti".format(p=target_path, i=i, w=w))
-        p = read_vti("{p}/lbmcl.{i:0{w}}.vti".format(p=prediction_path, i=i, w=w))
+        tpath = "{p}/ldc.0.{i:0{w}}.vti".format(p=target_path, i=i, w=w)
+        ppath = "{p}/lbmcl.{i:0{w}}.vti".format(p=prediction_path, i=i, w=w)
+        t, p = read_vti(tpath), read_vti(ppath)
+        if other_reader is not None:
+            cross_check(tpath, t, other_reader(tpath))
+            cross_check(ppath, p, other_reader(ppath))
         m = metrics(t["rho"], t["v"], p["rho"], p["v"])
         rows.append((i,) + m)
         print("{i:{w}}:  {0:e}   {1:e}   {2:e}   {3:e}".format(*m, i=i, w=w), file=out)
@@ -61,8 +90,14 @@ def main(argv=None):
     ap.add_argument("--max-mse-u", type=float, default=None)
     ap.add_argument("--max-mae-rho", type=float, default=None)
     ap.add_argument("--max-mae-u", type=float, default=None)
+    ap.add_argument("--pyvista", action="store_true", help="also read every file with pyvista and compare")
     a = ap.parse_args(argv)
-    rows = verify(a.iterations, a.every, a.target_path, a.prediction_path)
+    other = None
+    if a.pyvista:
+        other = pyvista_reader()
+        if other is None:
+            print("pyvista is not installed: cross-check skipped", file=sys.stderr)
+    rows = verify(a.iterations, a.every, a.target_path, a.prediction_path, other_reader=other)
     limits = (a.max_mse_rho, a.max_mse_u, a.max_mae_rho, a.max_mae_u)
     bad = [r for r in rows if any(lim is not None and v > lim for v, lim in zip(r[1:], limits))]
     if bad:

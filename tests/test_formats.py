@@ -149,3 +149,24 @@ def test_verify_reads_both_encodings(golden_dir, tmp_path):
         os.symlink(os.path.join(tdir, f"ldc.0.{i:02d}.vti"), tmp_path / f"lbmcl.{i:02d}.vti")
     rows = verify(10, 1, tdir, str(tmp_path), out=open(os.devnull, "w"))
     assert len(rows) == 11 and all(r[1:] == (0.0, 0.0, 0.0, 0.0) for r in rows)
+
+
+def test_verify_reader_cross_check(golden_dir, tmp_path, capsys):
+    """--pyvista: without pyvista the cross-check is reported as skipped; with a
+    second reader the arrays must agree, and a reader that disagrees fails."""
+    from lbmcl_amd import verify as V
+    tdir = os.path.join(golden_dir, "target_results", "8")
+    for i in (0, 5, 10):
+        os.symlink(os.path.join(tdir, f"ldc.0.{i:02d}.vti"), tmp_path / f"lbmcl.{i:02d}.vti")
+    if V.pyvista_reader() is None:
+        assert V.main(["-i", "10", "-e", "5", "-t", tdir, "-p", str(tmp_path), "--pyvista"]) == 0
+        assert "cross-check skipped" in capsys.readouterr().err
+    same = lambda path: read_vti(path)  # noqa: E731
+    assert len(V.verify(10, 5, tdir, str(tmp_path), out=open(os.devnull, "w"), other_reader=same)) == 3
+
+    def off(path):
+        d = read_vti(path)
+        return {"rho": d["rho"] + 1e-7, "v": d["v"]}
+
+    with pytest.raises(ValueError):
+        V.verify(10, 5, tdir, str(tmp_path), out=open(os.devnull, "w"), other_reader=off)
