@@ -126,7 +126,8 @@ def main():
     ap.add_argument("--dim", type=int, default=512)
     ap.add_argument("--stride", type=int, default=4096, help="CSoA stride")
     ap.add_argument("--variant-steps", type=int, default=300, help="steps of the FP16/BF16/FP64 runs (0: skip)")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="iterations of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=12,
+                    help="iterations of the CPU oracle sample, ~10 s on 16 threads (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-remap", action="store_true", help="disable the XCD-aware block order")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 row")

@@ -21,21 +21,23 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precision,kernel", [("f32", capi.LBM_KERNEL_VEC8), ("f64", capi.LBM_KERNEL_VEC),
-                                              ("f32", capi.LBM_KERNEL_GENERIC), ("f64", capi.LBM_KERNEL_ROW)])
+                                              ("f32", capi.LBM_KERNEL_GENERIC), ("f64", capi.LBM_KERNEL_ROW),
+                                              ("f32", capi.LBM_KERNEL_TILE), ("f64", capi.LBM_KERNEL_TILE)])
 def test_force_matches_oracle(precision, kernel):
     n = 128 if kernel != capi.LBM_KERNEL_GENERIC else 32
+    stride = 2 if kernel == capi.LBM_KERNEL_TILE else n
     dt = np.float32 if precision == "f32" else np.float64
     g = gaussian_random_field(n, 4.0, 1e-4, seed=5).astype(dt)
-    with capi.Context(n, stride=n, precision=precision, kernel=kernel) as a, \
-            capi.Context(n, stride=n, precision=precision, kernel=kernel) as b:
-        o = O.PullSim(n, stride=n, precision=precision)
+    with capi.Context(n, stride=stride, precision=precision, kernel=kernel) as a, \
+            capi.Context(n, stride=stride, precision=precision, kernel=kernel) as b:
+        o = O.PullSim(n, stride=stride, precision=precision)
         o.force = g.reshape(-1)
         a.set_force(np.zeros(3 * n ** 3, dtype=dt))
         b.set_force(g)
         a.initialize()
         b.initialize()
         o.initialize()
-        ref = capi.Context(n, stride=n, precision=precision, kernel=kernel)
+        ref = capi.Context(n, stride=stride, precision=precision, kernel=kernel)
         ref.initialize()
         for it in range(1, 7):
             a.compute(it, update_macro=True)
