@@ -455,12 +455,14 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Vector row kernel.  Each lane owns V = LB / sizeof(S) consecutive cells of a
 // row (LB = 16 or 8 bytes: 4 or 2 cells in FP32, 2 or 1 in FP64, 8 or 4 with
-// 16-bit storage), so every population moves as one LB-byte access per lane: an aligned vector load of the lane's own
-// slots, and for populations with e_x = +-1 the one element that crosses into
-// the neighbouring lane comes from that lane by a cross-lane shuffle (a
-// wavefront-uniform load at the two wavefront ends).  Stores are whole aligned
-// vectors: the two wall cells of a row (x = 0, N-1) get their initial pull-form
-// values back, which is what they hold already, so wall slots keep P0.
+// 16-bit storage), so every population moves as one LB-byte access per lane:
+// an aligned vector load of the lane's own slots, and for populations with
+// e_x = +-1 the one element that crosses into the neighbouring lane comes from
+// that lane by a cross-lane shuffle; at a wavefront's ends it comes from the
+// neighbouring wavefront of the block through LDS, and at the block's ends by
+// a wavefront-uniform load.  Stores are whole aligned vectors: the two wall
+// cells of a row (x = 0, N-1) get their initial pull-form values back, which
+// is what they hold already, so wall slots keep P0.
 // Requirements: stride >= V (V consecutive cells are contiguous and LB-byte
 // aligned for every population) and N / V >= 64 (a wavefront lies in one row).
 // ---------------------------------------------------------------------------
@@ -555,12 +557,41 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     SV A[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) A[i] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(src_row(i) + off0));
-    S edge[Q];
+    // wavefront-end elements: from the neighbouring wavefront of the block
+    // through LDS, from memory only at the block's ends
+    __shared__ S xe_up[16][5];  // per wavefront: lane 63's last element, e_x = +1 populations
+    __shared__ S xe_dn[16][5];  // per wavefront: lane 0's first element, e_x = -1 populations
+    const uint32_t wave = tid >> 6, lane = tid & 63u, nwaves = blockDim.x >> 6;
+    {
+        int ku = 0, kd = 0;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        edge[i] = S(0);
-        if (ex_of(i) == 1 && has_left) edge[i] = src_row(i)[voff(xw - 1)];
-        if (ex_of(i) == -1 && has_right) edge[i] = src_row(i)[voff(xw + 64 * V)];
+        for (int i = 0; i < Q; ++i) {
+            if (ex_of(i) == 1) {
+                if (lane == 63) xe_up[wave][ku] = A[i][V - 1];
+                ++ku;
+            } else if (ex_of(i) == -1) {
+                if (lane == 0) xe_dn[wave][kd] = A[i][0];
+                ++kd;
+            }
+        }
+    }
+    __syncthreads();
+    S edge[Q];
+    {
+        int ku = 0, kd = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            edge[i] = S(0);
+            if (ex_of(i) == 1) {
+                if (wave > 0) edge[i] = xe_up[wave - 1][ku];
+                else if (has_left) edge[i] = src_row(i)[voff(xw - 1)];
+                ++ku;
+            } else if (ex_of(i) == -1) {
+                if (wave + 1 < nwaves) edge[i] = xe_dn[wave + 1][kd];
+                else if (has_right) edge[i] = src_row(i)[voff(xw + 64 * V)];
+                ++kd;
+            }
+        }
     }
     // gather: g[i][j] is population i arriving at cell x0 + j
     S g[Q][V];
