@@ -297,6 +297,8 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
     typedef T SV __attribute__((ext_vector_type(V)));
     __shared__ T xup[8][5];  // per wavefront: lane 0's lam_i of its first cell, e_x(i) = +1
     __shared__ T xdn[8][5];  // per wavefront: lane 63's lam_i of its last cell, e_x(i) = -1
+    __shared__ T pe_up[8][5];  // per wavefront: lane 63's last gathered-row element, e_x(i) = +1
+    __shared__ T pe_dn[8][5];  // per wavefront: lane 0's first gathered-row element, e_x(i) = -1
     __shared__ double lds[3 * 8];
     const int32_t n = a.n;
     const uint32_t rows = uint32_t(n - 2);
@@ -323,8 +325,8 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
     const int64_t oc0 = int64_t(n) * r;
     const int64_t own = 19 * oc0 - 18 * (oc0 & smask);
     const uint32_t off0 = voff(x0);
-    const bool has_left = xw > 0;
-    const bool has_right = xw + 64 * V < n;
+    const bool has_left = xw > 0;            // wave > 0 (one block per row)
+    const bool has_right = xw + 64 * V < n;  // wave < nwaves - 1
 
     SV A[Q];
 #pragma unroll
@@ -340,12 +342,36 @@ __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
             Lv[j] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(a.Lnext + own + int64_t(j) * a.stride + off0));
         }
     }
-    T edge[Q];
+    // wavefront-end elements of the gather from the neighbouring wavefront
+    // (the block is the whole row: the row's ends are walls, which need none)
+    {
+        int ku = 0, kd = 0;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        edge[i] = T(0);
-        if (ex_of(i) == 1 && has_left) edge[i] = a.P[row_slot(i) + voff(xw - 1)];
-        if (ex_of(i) == -1 && has_right) edge[i] = a.P[row_slot(i) + voff(xw + 64 * V)];
+        for (int i = 0; i < Q; ++i) {
+            if (ex_of(i) == 1) {
+                if (lane == 63) pe_up[wave][ku] = A[i][V - 1];
+                ++ku;
+            } else if (ex_of(i) == -1) {
+                if (lane == 0) pe_dn[wave][kd] = A[i][0];
+                ++kd;
+            }
+        }
+    }
+    __syncthreads();
+    T edge[Q];
+    {
+        int ku = 0, kd = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            edge[i] = T(0);
+            if (ex_of(i) == 1) {
+                if (has_left) edge[i] = pe_up[wave - 1][ku];
+                ++ku;
+            } else if (ex_of(i) == -1) {
+                if (has_right) edge[i] = pe_dn[wave + 1][kd];
+                ++kd;
+            }
+        }
     }
     T g[Q][V];
 #pragma unroll
