@@ -284,11 +284,12 @@ inline bool vec_possible(const lbm_ctx* c, KernelKind k) {
     return v >= 1 && c->stride >= v && c->n / v >= 64;
 }
 
-// Preferred vector width per storage type (measured on MI355X at 512^3):
-// 16-byte lanes for FP64 (2 cells per lane), 8-byte lanes for FP32 and 16-bit
-// storage, whose 16-byte forms need too many registers for the occupancy the
-// kernel wants.
-inline KernelKind preferred_vec(const lbm_ctx* c) { return c->ssize == 8 ? K_VEC : K_VEC8; }
+// Preferred vector width (measured on MI355X at 512^3, stride 4096): 8-byte
+// lanes for every storage type.  16-byte lanes need too many registers for
+// four waves per SIMD in FP32 and 16-bit storage, and in FP64 (two cells per
+// lane) they were 1-2.5% slower than one cell per lane once the wavefront ends
+// came through LDS.
+inline KernelKind preferred_vec(const lbm_ctx*) { return K_VEC8; }
 
 inline KernelKind kernel_kind(const lbm_ctx* c) {
     switch (c->cfg.kernel) {
