@@ -65,6 +65,7 @@ CASES = [
     (128, 2, "f64", capi.LBM_KERNEL_VEC, 6),
     (128, 128, "f64", capi.LBM_KERNEL_VEC, 6),
     (512, 1024, "f64", capi.LBM_KERNEL_VEC, 2),
+    (512, 4096, "f64", capi.LBM_KERNEL_VEC8, 2),  # two blocks per row: block-end elements from memory
     (256, 2, "f32", capi.LBM_KERNEL_VEC8, 5),
     (128, 128, "f32", capi.LBM_KERNEL_VEC8, 6),
     (64, 64 ** 3, "f64", capi.LBM_KERNEL_VEC8, 8),
