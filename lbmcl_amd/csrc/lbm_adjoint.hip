@@ -36,13 +36,16 @@ struct AdjointBuffers {
     }
 };
 
-// The vector adjoint kernel needs whole wavefronts per row (N / V >= 64), at
-// most 512 threads per row, and V consecutive cells contiguous (stride >= V).
-// LBM_KERNEL_GENERIC and LBM_KERNEL_ROW select the generic adjoint kernel.
-template <class T> bool adjoint_vec_possible(const lbm_ctx* c) {
-    const int64_t V = lbm::AdjVec<T>::V;
-    if (c->cfg.kernel == LBM_KERNEL_GENERIC || c->cfg.kernel == LBM_KERNEL_ROW) return false;
-    return c->n / V >= 64 && c->n / V <= 512 && c->stride >= V;
+// Cells per lane of the vector adjoint kernel, 0 if it cannot run: it needs
+// whole wavefronts per row (N / V >= 64), at most 512 threads per row, and V
+// consecutive cells contiguous (stride >= V).  8-byte lanes where possible
+// (FP32: V = 2), else one cell per lane.  LBM_KERNEL_GENERIC and
+// LBM_KERNEL_ROW select the generic adjoint kernel.
+template <class T> int64_t adjoint_vec_width(const lbm_ctx* c) {
+    if (c->cfg.kernel == LBM_KERNEL_GENERIC || c->cfg.kernel == LBM_KERNEL_ROW) return 0;
+    for (int64_t V : {int64_t(8 / sizeof(T)), int64_t(1)})
+        if (c->n / V >= 64 && c->n / V <= 512 && c->stride >= V) return V;
+    return 0;
 }
 
 template <class T>
@@ -99,8 +102,8 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         }
     }
     // vector adjoint kernel: one block of N / V threads per interior row
-    const bool vec = adjoint_vec_possible<T>(c);
-    const int64_t V = lbm::AdjVec<T>::V;
+    const int64_t V = adjoint_vec_width<T>(c);
+    const bool vec = V > 0;
     const int block = vec ? int(c->n / V) : 256;
     const int64_t nblocks = vec ? int64_t(c->n - 2) * (c->n - 2) : (c->n3 + 255) / 256;
     double* partials = nullptr;
@@ -184,8 +187,12 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
         a.Lprev = static_cast<T*>(Lprev);
         a.seed = seed;
         a.first = first;
-        if (vec)
-            hipLaunchKernelGGL((lbm::lbm_adjoint_vec<T>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
+        if (vec && V == 2) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL((lbm::lbm_adjoint_vec<T, 2>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream,
+                                   a);
+        } else if (vec)
+            hipLaunchKernelGGL((lbm::lbm_adjoint_vec<T, 1>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
         else
             hipLaunchKernelGGL((lbm::lbm_adjoint_step<T>), dim3(unsigned(nblocks)), dim3(block), 0, c->stream, a);
         LBM_HIP_CHECK(hipGetLastError(), "adjoint step launch");

@@ -275,11 +275,12 @@ __global__ __launch_bounds__(256) void lbm_adjoint_step(AdjArgs<T> a) {
     block_accumulate3(dU, dW, J, lds, a.partials + 3 * blockIdx.x);
 }
 
-// Vector adjoint step (FP32: 2 cells per lane, FP64: 1).  One block per
-// interior row (y, z), N / V threads, so every cross-lane move stays inside
-// the block.  The gather of P_k is the forward vector kernel's (aligned 8-byte
-// loads, DPP shifts by one lane, wavefront-uniform loads at the wavefront
-// ends); lambda_{k+1} is read at the lane's own aligned slots.  The scatter
+// Vector adjoint step (V = 2 cells per lane in FP32 when the row has at least
+// 128 cells and the stride is at least 2, else 1).  One block per interior row
+// (y, z), N / V threads, so every cross-lane move stays inside the block.  The
+// gather of P_k is the forward vector kernel's (aligned V-element loads, DPP
+// shifts by one lane, wavefront ends through LDS); lambda_{k+1} is read at the
+// lane's own aligned slots.  The scatter
 // lambda_k[d - e_i][i] = lam_i(d) is the gather reversed: the vector written at
 // the lane's aligned position of population i's source row holds lam_i of the
 // cells x + e_x, moved by one lane the other way, and the element that crosses
@@ -287,13 +288,8 @@ __global__ __launch_bounds__(256) void lbm_adjoint_step(AdjArgs<T> a) {
 // cells of the row scatter zeros, which is what the slots they feed must hold;
 // slots fed by wall rows are never written and keep the zeros both adjoint
 // lattices start with.
-template <class T> struct AdjVec {
-    static constexpr int V = int(8 / sizeof(T));
-};
-
-template <class T>
+template <class T, int V>
 __global__ __launch_bounds__(512) void lbm_adjoint_vec(AdjArgs<T> a) {
-    constexpr int V = AdjVec<T>::V;
     typedef T SV __attribute__((ext_vector_type(V)));
     __shared__ T xup[8][5];  // per wavefront: lane 0's lam_i of its first cell, e_x(i) = +1
     __shared__ T xdn[8][5];  // per wavefront: lane 63's lam_i of its last cell, e_x(i) = -1

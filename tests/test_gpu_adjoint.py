@@ -65,12 +65,13 @@ def _adjoint_directional(r, U, nu, dU, dnu, dg):
 
 
 @pytest.mark.parametrize("precision,n,stride", [("f32", 128, 128), ("f32", 128, 2), ("f64", 64, 64),
-                                               ("f64", 128, 16)])
+                                               ("f64", 128, 16), ("f32", 64, 64), ("f32", 128, 1)])
 @pytest.mark.parametrize("objective", ["kinetic_energy", "mean_ux_plane"])
 def test_vector_adjoint_matches_generic(precision, n, stride, objective):
-    """The vector adjoint kernel (one block per row, DPP/LDS scatter) against the
-    generic one-thread-per-cell kernel: per-cell gradients bit-identical, the
-    FP64 sums equal up to their summation order; checkpointed == full tape."""
+    """The vector adjoint kernel (one block per row, DPP/LDS scatter; two cells
+    per lane in FP32 at N >= 128 and stride >= 2, else one) against the generic
+    one-thread-per-cell kernel: per-cell gradients bit-identical, the FP64 sums
+    equal up to their summation order; checkpointed == full tape."""
     steps = 30
     g = gaussian_random_field(n, 8.0, 1e-6, seed=7)
     kw = dict(force=g, objective=objective, precision=precision, stride=stride)
