@@ -5,8 +5,10 @@
 // stream), which is the practical ceiling for any streaming kernel, and as the
 // calibration point of the rocprofv3 FETCH_SIZE / WRITE_SIZE counters (a copy
 // of B bytes must fetch and write B bytes).  One 16-byte element per thread,
-// streaming (nontemporal) loads and stores: the fastest copy shape measured on
-// MI355X (6.5 TB/s; cached loads 6.33, four elements per thread 6.0).
+// streaming (nontemporal) loads and stores, XCD-contiguous block order: the
+// fastest copy shape measured on MI355X (6.6 TB/s; plain block order 6.5,
+// cached loads 6.33, four elements per thread 6.0; read-only 7.1, write-only
+// 6.8).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -15,8 +17,15 @@ namespace {
 
 typedef uint32_t U4 __attribute__((ext_vector_type(4)));
 
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of blocks (the same order the LBM kernels use).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, xcd = b & 7u, k = b >> 3;
+    return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(const U4* __restrict__ src, U4* __restrict__ dst, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t i = int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
