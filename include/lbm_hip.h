@@ -57,8 +57,8 @@ extern "C" {
  * (VEC8) bytes of every population: V = lane bytes / bytes per stored
  * population consecutive cells; they need stride >= V and dim / V >= 64.
  * LBM_KERNEL_AUTO picks the tile kernel for small strides (below), else VEC8
- * when possible, else the row kernel when dim >= 64, else the generic kernel.
- * All kernels compute identical results. */
+ * when possible, else the tile kernel when possible, else the row kernel when
+ * dim >= 64, else the generic kernel.  All kernels compute identical results. */
 #define LBM_KERNEL_AUTO 0
 #define LBM_KERNEL_GENERIC 1
 #define LBM_KERNEL_ROW 2

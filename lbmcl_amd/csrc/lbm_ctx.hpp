@@ -303,6 +303,9 @@ inline KernelKind kernel_kind(const lbm_ctx* c) {
             const KernelKind p = preferred_vec(c);
             if (vec_possible(c, p)) return p;
             if (vec_possible(c, K_VEC8)) return K_VEC8;
+            // rows too short for whole wavefronts of vectors (e.g. 16-bit
+            // storage below 256^3): the tile kernel when the stride allows it
+            if (tile_possible(c)) return K_TILE;
             return c->n >= 64 ? K_ROW : K_GENERIC;
         }
     }
