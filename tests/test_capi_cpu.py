@@ -128,3 +128,14 @@ def test_cli_without_device_asks_then_fails(lbmcl_bin):
     r = run_cli(lbmcl_bin, "-d", "8", "-D", "0")
     assert "Select a device" in r.stdout
     assert r.returncode != 0 and "no device selected" in r.stderr
+
+
+def test_c_example_builds_and_reports_missing_device(lbmcl_bin):
+    """csrc/examples/capi_example.c compiles as C99 against the header and links the library;
+    without a device lbm_create fails with lbm_last_error's message."""
+    exe = native_build.CAPI_EXAMPLE
+    assert os.path.exists(exe)
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is present")
+    r = run_cli(exe, "16", "2")
+    assert r.returncode == 1 and "no HIP device" in r.stderr

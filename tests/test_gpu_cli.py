@@ -102,3 +102,24 @@ def test_cli_fp16_storage_runs(tmp_path):
     for _ in range(20):
         sim.step(True)
     assert np.array_equal(vt["rho"], window(sim.rho, 64), equal_nan=True)
+
+
+def test_capi_example_c_caller():
+    """csrc/examples/capi_example.c: the C ABI from plain C, checked against the oracle."""
+    exe = native_build.CAPI_EXAMPLE
+    n, iters = 64, 100
+    print(exe, n, iters, flush=True)
+    r = subprocess.run([exe, str(n), str(iters)], capture_output=True, text=True, timeout=300,
+                       stdin=subprocess.DEVNULL)
+    assert r.returncode == 0, r.stderr
+    words = r.stdout.split()
+    rho_c = float(words[words.index("rho(centre)") + 1])
+    mass = float(words[words.index("mass") + 1])
+    o = O.PullSim(n, stride=64)
+    o.initialize()
+    for _ in range(iters):
+        o.step(update_macro=True)
+    w = n - 2
+    centre = (w // 2) + w * ((w // 2) + w * (w // 2))
+    assert abs(rho_c - float(window(o.rho, n)[centre])) <= 6e-8, r.stdout
+    assert abs(mass - o.diag[0]) <= 1e-5, (r.stdout, o.diag)
