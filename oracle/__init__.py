@@ -24,6 +24,11 @@ Pinning (what checks this oracle, see tests/test_oracle.py):
   diagnostics have no reference counterpart; they are pinned by their
   mathematics (zero force is bit-identical to no force; the force term is
   linear and conserves mass; diagnostics against numpy sums).
+
+``ref_run`` runs the reference's own kernels.cl -- compiled offline from the
+reference's source by ``ref_build.py`` into ``oracle/_ref`` -- on the local GPU
+through the ROCm OpenCL runtime (``ref_runner.c``), for comparisons with "the
+reference run at the same precision" and for its throughput on the same GPU.
 """
 from __future__ import annotations
 
@@ -62,8 +67,12 @@ class Params(ctypes.Structure):
 
 
 def build() -> None:
-    """Compile liblbm_oracle.so with the recipe in oracle/Makefile."""
+    """Compile liblbm_oracle.so with the recipe in oracle/Makefile and, where the
+    reference checkout is present, oracle/_ref with oracle/ref_build.py."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+    from . import ref_build
+
+    ref_build.build(quiet=True)
 
 
 _lib = None
@@ -295,4 +304,51 @@ class PullSim:
 def diagnostics(p: Params, rho, u, precision="f32"):
     out = np.zeros(2, dtype=np.float64)
     getattr(lib(), f"lbmo_diagnostics_{precision}")(ctypes.byref(p), _ptr(rho), _ptr(u), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# The reference's own kernels on the GPU (oracle/_ref, see ref_build.py).
+# ---------------------------------------------------------------------------
+def ref_unavailable_reason():
+    """None when oracle/_ref is built and an OpenCL GPU device is present, else why not."""
+    from . import ref_build
+
+    if not os.path.exists(ref_build.RUNNER):
+        return "oracle/_ref is not built (it needs the reference checkout at build time)"
+    r = subprocess.run([ref_build.RUNNER, "--probe"], capture_output=True, text=True, timeout=120,
+                       stdin=subprocess.DEVNULL)
+    if r.returncode != 0:
+        return f"no OpenCL GPU device ({r.stdout.strip()} {r.stderr.strip()})"
+    return None
+
+
+def ref_run(precision, dim, stride, lws, iterations, out_prefix=None, every=None, skip=0, fast=False,
+            timeout=600):
+    """Run the reference's initialize + `iterations` compute kernels (lbmcl.hpp:490-521).
+
+    `every` defaults to `iterations` (rho/u stored by the last iteration).  Returns
+    {"mlups", "kernels_ms", "device"} and, with `out_prefix`, the arrays "rho" (N^3),
+    "u" (3 x N^3) and "f" (19 N^3, CSoA, the lattice iteration iterations + 1 reads)."""
+    from . import ref_build
+
+    co = ref_build.co_path(precision, dim, stride, fast)
+    if not os.path.exists(co):
+        raise FileNotFoundError(f"{co}: no reference build for this configuration (see ref_build.CONFIGS)")
+    every = iterations if every is None else every
+    cmd = [ref_build.RUNNER, co, precision, str(dim), str(stride), ",".join(map(str, lws)), str(iterations),
+           str(every), str(skip), out_prefix or "-"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, stdin=subprocess.DEVNULL)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stderr.strip()}")
+    words = r.stdout.split()
+    out = {"mlups": float(words[words.index("mlups") + 1]),
+           "kernels_ms": float(words[words.index("kernels_ms") + 1]),
+           "device": words[1]}
+    if out_prefix:
+        dt = np.float64 if precision == "f64" else np.float32
+        n3 = dim ** 3
+        out["rho"] = np.fromfile(out_prefix + ".rho", dtype=dt)
+        out["u"] = np.fromfile(out_prefix + ".u", dtype=dt).reshape(3, n3)
+        out["f"] = np.fromfile(out_prefix + ".f", dtype=dt)
     return out

@@ -58,6 +58,8 @@ def test_library_exports_every_declared_symbol():
     (dict(dim=8, viscosity=-1.0), capi.LBM_ERR_INVALID, "viscosity"),
     (dict(dim=8, velocity=float("nan")), capi.LBM_ERR_INVALID, "velocity"),
     (dict(dim=8, lws=(64, 64, 1)), capi.LBM_ERR_INVALID, "work-group"),
+    (dict(dim=8, optimize=3), capi.LBM_ERR_INVALID, "LBM_ARITH_OPENCL"),
+    (dict(dim=8, storage="bf16", optimize=capi.LBM_ARITH_OPENCL), capi.LBM_ERR_INVALID, "native storage"),
     (dict(dim=8, device=99), capi.LBM_ERR_DEVICE, "no HIP device"),
 ])
 def test_create_rejects_bad_configurations(kw, code, msg):

@@ -1,0 +1,196 @@
+"""The HIP path against the reference's own kernels.cl run on the same GPU.
+
+oracle/_ref holds kernels.cl compiled offline (oracle/ref_build.py, the options
+of lbmcl.hpp:131-156) and a runner that drives it through the ROCm OpenCL
+runtime with the host flow of lbmcl.hpp:392-521 (oracle/ref_runner.c).  This is
+"the reference run at the same precision".
+
+* LBM_ARITH_OPENCL evaluates kernels.cl's expressions the way ROCm's OpenCL
+  compiler does (multiply-adds contracted as clang forms them, FP32 division
+  to 2.5 ulp): rho, u and the full CSoA lattice that the next iteration reads
+  are bit-identical to the reference's, for every kernel form.
+* LBM_ARITH_EXACT (the default, bit-identical to the CPU oracle) evaluates
+  every operation as written.  Against the reference it differs by FP32/FP64
+  rounding only: within 1e-12 relative in FP64; in FP32 the difference grows
+  with the number of steps, so it is bounded by the reference's own FP32
+  rounding error -- measured against the reference run in FP64 -- instead.
+
+ROCm's OpenCL limits a work-group to 256 work-items, so the make test8 and
+test32 work-groups (8,8,8) and (32,32,1) (Makefile:77, 85) are replaced by
+(8,8,4) and (32,8,1); no result depends on the work-group size.
+
+The throughput test runs the reference's kernels over a sweep of strides and
+work-group sizes at 256^3 (the largest lattice its int indexing allows) and
+requires the HIP path to be faster than the best of them.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbmcl_amd import capi
+from oracle import ref_build
+from tests.util import max_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 1e-5, "f64": 1e-12}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def reference_available():
+    reason = O.ref_unavailable_reason()
+    if reason:
+        pytest.skip(f"the reference's kernels cannot run here: {reason}")
+
+
+def _compare(ours, ref, tol, label, mask=None):
+    a, b = np.asarray(ours, dtype=np.float64), np.asarray(ref, dtype=np.float64)
+    if mask is not None:
+        a, b = a[..., mask], b[..., mask]
+    assert np.array_equal(np.isnan(a), np.isnan(b)), f"{label}: NaN patterns differ"
+    rel = max_rel(a, b)
+    print(f"{label}: max relative difference {rel:.3e} (limit {tol:g})", flush=True)
+    assert rel <= tol, f"{label}: {rel:.3e} > {tol:g}"
+    return rel
+
+
+def _hip_run(precision, n, stride, iterations, arith=capi.LBM_ARITH_EXACT, kernel=capi.LBM_KERNEL_AUTO):
+    g = capi.Context(n, stride=stride, precision=precision, optimize=arith, kernel=kernel)
+    with g:
+        g.initialize()
+        if iterations > 1:
+            g.compute_range(1, iterations - 1, 0)
+        g.compute(iterations, update_macro=True)
+        rho, u = g.read_rho(), g.read_u().reshape(3, n ** 3)
+        # the lattice that iteration iterations + 1 reads (lbmcl.hpp:518)
+        f = g.read_f(1 if (iterations + 1) % 2 == 0 else 0)
+    return rho, u, f
+
+
+CASES = [
+    # make test8 and make test32 shapes (Makefile:73-86), 64^3, and the 128^3 x 1000 configuration
+    ("f32", 8, 8, (8, 8, 4), 10),
+    ("f64", 8, 8, (8, 8, 4), 10),
+    ("f32", 32, 32, (32, 8, 1), 500),
+    ("f64", 32, 32, (32, 8, 1), 500),
+    ("f32", 64, 64, (64, 1, 1), 300),
+    ("f64", 64, 64, (64, 1, 1), 300),
+    ("f32", 128, 64, (64, 1, 1), 1000),
+    ("f64", 128, 64, (64, 1, 1), 1000),
+]
+
+
+def _bits_equal(ours, ref, label):
+    a, b = np.asarray(ours), np.asarray(ref)
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    bad = int(np.count_nonzero(~same))
+    print(f"{label}: {bad} of {a.size} values differ from the reference", flush=True)
+    assert bad == 0, f"{label}: {bad} values differ (max relative {max_rel(a, b):.3e})"
+
+
+@pytest.mark.parametrize("precision,n,stride,lws,iterations", CASES)
+def test_opencl_arithmetic_is_bit_identical_to_reference_kernels(tmp_path, precision, n, stride, lws, iterations):
+    label = f"OpenCL arithmetic {n}^3 {precision} stride {stride} x {iterations}"
+    print(f"reference kernels.cl vs HIP: {label}", flush=True)
+    ref = O.ref_run(precision, n, stride, lws, iterations, out_prefix=str(tmp_path / "ref"))
+    rho, u, f = _hip_run(precision, n, stride, iterations, arith=capi.LBM_ARITH_OPENCL)
+    _bits_equal(rho, ref["rho"], f"{label} rho")
+    _bits_equal(u, ref["u"], f"{label} u")
+    _bits_equal(f, ref["f"], f"{label} f")
+
+
+@pytest.mark.parametrize("precision,n,stride,kernel", [
+    ("f32", 64, 64, capi.LBM_KERNEL_GENERIC), ("f32", 64, 64, capi.LBM_KERNEL_ROW),
+    ("f32", 256, 64, capi.LBM_KERNEL_VEC), ("f32", 64, 1, capi.LBM_KERNEL_TILE),
+    ("f64", 64, 64, capi.LBM_KERNEL_GENERIC), ("f64", 64, 64, capi.LBM_KERNEL_ROW),
+    ("f64", 64, 1, capi.LBM_KERNEL_TILE),
+])
+def test_opencl_arithmetic_every_kernel_form(tmp_path, precision, n, stride, kernel):
+    """The generic, row, 16-byte vector and tile kernels in the OpenCL form (the
+    cases above run the automatic choice: 8-byte vector or tile)."""
+    iterations = 100 if n <= 64 else 20
+    label = f"OpenCL arithmetic {n}^3 {precision} stride {stride} kernel {kernel} x {iterations}"
+    print(f"reference kernels.cl vs HIP: {label}", flush=True)
+    ref = O.ref_run(precision, n, stride, (64, 1, 1), iterations, out_prefix=str(tmp_path / "ref"))
+    rho, u, f = _hip_run(precision, n, stride, iterations, arith=capi.LBM_ARITH_OPENCL, kernel=kernel)
+    _bits_equal(rho, ref["rho"], f"{label} rho")
+    _bits_equal(u, ref["u"], f"{label} u")
+    _bits_equal(f, ref["f"], f"{label} f")
+
+
+@pytest.mark.parametrize("precision,n,stride,lws,iterations", CASES)
+def test_exact_arithmetic_against_reference_kernels(tmp_path, precision, n, stride, lws, iterations):
+    label = f"exact arithmetic {n}^3 {precision} stride {stride} x {iterations}"
+    print(f"reference kernels.cl vs HIP: {label}", flush=True)
+    ref = O.ref_run(precision, n, stride, lws, iterations, out_prefix=str(tmp_path / "ref"))
+    rho, u, f = _hip_run(precision, n, stride, iterations)
+    if precision == "f64":
+        _compare(rho, ref["rho"], TOL["f64"], f"{label} rho")
+        _compare(u, ref["u"], TOL["f64"], f"{label} u")
+        _compare(f, ref["f"], TOL["f64"], f"{label} f")
+        return
+    # FP32: within the reference's own FP32 rounding error (its FP32 run
+    # against its FP64 run), and never more than 1e-4 relative
+    ref64 = O.ref_run("f64", n, stride, lws, iterations, out_prefix=str(tmp_path / "ref64"))
+    for name, ours, theirs, exact in (("rho", rho, ref["rho"], ref64["rho"]), ("u", u, ref["u"], ref64["u"])):
+        own = max_rel(theirs, exact)
+        print(f"{label} {name}: reference FP32 vs reference FP64 {own:.3e}; HIP vs reference FP64 "
+              f"{max_rel(ours, exact):.3e}", flush=True)
+        _compare(ours, theirs, max(TOL["f32"], min(2 * own, 1e-4)), f"{label} {name}")
+        assert max_rel(ours, exact) <= 1.5 * own + 1e-7, f"{label} {name}: further from FP64 than the reference"
+
+
+@pytest.mark.parametrize("n,stride,lws,iterations", [(32, 32, (32, 8, 1), 500), (128, 64, (64, 1, 1), 1000)])
+def test_fast_math_matches_reference_fast_math(tmp_path, n, stride, lws, iterations):
+    """-o: the reference's -cl-fast-relaxed-math build against the fused collision.
+    Under finite-math-only the reference's NaN stores are undefined, so rho and u
+    are compared on the cells that store them (FLUID and MOVING, kernels.cl:382)."""
+    label = f"-o {n}^3 f32 stride {stride} x {iterations}"
+    print(f"reference kernels.cl vs HIP: {label}", flush=True)
+    ref = O.ref_run("f32", n, stride, lws, iterations, out_prefix=str(tmp_path / "ref"), fast=True)
+    rho, u, _ = _hip_run("f32", n, stride, iterations, arith=capi.LBM_ARITH_FUSED)
+    stored = np.isfinite(rho)
+    _compare(rho, ref["rho"], 1e-4, f"{label} rho", mask=stored)
+    _compare(u, ref["u"], 1e-4, f"{label} u", mask=stored)
+
+
+def _hip_mlups(precision, n, stride, steps, warmup, arith=capi.LBM_ARITH_EXACT):
+    g = capi.Context(n, stride=stride, precision=precision, optimize=arith)
+    with g:
+        g.initialize()
+        g.compute_range(1, warmup, 0)
+        g.finish()
+        g.clear_events()
+        g.compute_range(warmup + 1, steps, 0)
+        g.finish()
+        ms = g.kernels_time_ms()
+    return (n - 2) ** 3 * steps / (ms * 1e3)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_faster_than_reference_kernels(precision):
+    """256^3: the reference's compute kernel over strides 1, 64, 4096, N^3 and
+    work-groups (64,1,1), (256,1,1) -- plus its -o build for FP32 -- against the
+    HIP path at its default stride, kernel MLUPS from profiling events over
+    200 iterations after 20 (the reference's kernelsMLUPS, lbmcl.hpp:606-619)."""
+    n, steps, warmup = 256, 200, 20
+    rows = []
+    fasts = (False, True) if precision == "f32" else (False,)
+    for fast in fasts:
+        for stride in (1, 64, 4096, n ** 3):
+            if not os.path.exists(ref_build.co_path(precision, n, stride, fast)):
+                continue
+            for lws in ((64, 1, 1), (256, 1, 1)):
+                r = O.ref_run(precision, n, stride, lws, warmup + steps, every=0, skip=warmup, fast=fast)
+                rows.append((r["mlups"], f"reference kernels.cl{' -o' if fast else ''} stride {stride} lws {lws}"))
+                print(f"{n}^3 {precision} {rows[-1][1]}: {r['mlups']:.0f} MLUPS", flush=True)
+    best_ref = max(rows)
+    ours = _hip_mlups(precision, n, 4096, steps, warmup)
+    ours_fast = _hip_mlups(precision, n, 4096, steps, warmup, arith=capi.LBM_ARITH_FUSED)
+    ours_ocl = _hip_mlups(precision, n, 4096, steps, warmup, arith=capi.LBM_ARITH_OPENCL)
+    print(f"{n}^3 {precision} HIP stride 4096: exact {ours:.0f} MLUPS, fused (-o) {ours_fast:.0f}, OpenCL "
+          f"arithmetic {ours_ocl:.0f}; best reference {best_ref[0]:.0f} ({best_ref[1]}); ratio "
+          f"{ours / best_ref[0]:.2f}", flush=True)
+    assert min(ours, ours_fast, ours_ocl) > best_ref[0]
