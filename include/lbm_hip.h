@@ -70,6 +70,11 @@ extern "C" {
  * LBM_KERNEL_AUTO selects it when stride * stored bytes <= 32. */
 #define LBM_KERNEL_TILE 5
 
+/* lbm_config.optimize: arithmetic form of the collision (see lbm_compute) */
+#define LBM_ARITH_EXACT 0   /* kernels.cl's expressions operation by operation, no contraction */
+#define LBM_ARITH_FUSED 1   /* -o: the fused BGK form (always used with 16-bit storage) */
+#define LBM_ARITH_OPENCL 2  /* kernels.cl's expressions as ROCm's OpenCL compiler evaluates them */
+
 /* event kinds, named like the reference's events (lbmcl.hpp:26-31) */
 #define LBM_EVENT_INITIALIZE 0
 #define LBM_EVENT_COMPUTE 1
@@ -88,7 +93,7 @@ typedef struct lbm_config {
     double viscosity;    /* kinematic viscosity nu; tau = 3 nu + 1/2 (kernels.cl:61-62) */
     double velocity;     /* lid x-velocity U (kernels.cl:57) */
     int32_t lws[3];      /* work-group size of the generic kernel (product <= 1024); 0 = default */
-    int32_t optimize;    /* -o: fused collision (see lbm_compute); 16-bit storage always uses it */
+    int32_t optimize;    /* LBM_ARITH_*: 1 is -o (fused collision; 16-bit storage always uses it) */
     int32_t kernel;      /* LBM_KERNEL_* */
     int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
     int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernels) */
@@ -149,12 +154,18 @@ int lbm_initialize(lbm_ctx* ctx);
  * diagnostics != 0 also reduces the mass sum(rho) and kinetic energy
  * sum(rho |u|^2 / 2) over FLUID/MOVING cells in FP64; read them with
  * lbm_read_diagnostics (no counterpart in the reference).
- * Arithmetic: by default each expression of kernels.cl:343-418 is evaluated
- * operation by operation without contraction.  With optimize != 0 (the -o
- * option, the reference's -cl-fast-relaxed-math) and always for 16-bit
- * storage, the BGK update uses a fused form, f' = (1 - 1/tau) f + (rho w_i / tau) T_i
- * with T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u, evaluated with
- * fused multiply-adds; rho and u are unchanged, results differ by rounding. */
+ * Arithmetic (lbm_config.optimize): LBM_ARITH_EXACT evaluates each expression
+ * of kernels.cl:343-418 operation by operation without contraction.
+ * LBM_ARITH_FUSED (the -o option, the reference's -cl-fast-relaxed-math, and
+ * always for 16-bit storage) uses a fused BGK form, f' = (1 - 1/tau) f +
+ * (rho w_i / tau) T_i with T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u,
+ * evaluated with fused multiply-adds; rho and u are unchanged, results differ
+ * by rounding.  LBM_ARITH_OPENCL (native storage only) evaluates the same
+ * expressions as ROCm's OpenCL compiler builds kernels.cl without -o: products
+ * added or subtracted within one expression are contracted into fused
+ * multiply-adds (OpenCL C's default FP_CONTRACT ON, as clang forms them) and
+ * FP32 divisions round to OpenCL's 2.5 ulp; lbm_initialize then uses the same
+ * contracted equilibrium.  This is the reference's own arithmetic on this GPU. */
 int lbm_compute(lbm_ctx* ctx, int64_t iteration, int32_t update_macro, int32_t diagnostics);
 
 /* Enqueue iterations first .. first+count-1; update_macro is set on those

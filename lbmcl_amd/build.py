@@ -45,9 +45,18 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+OBJ_DIR = os.path.join(PKG, "_build")
+
+
 def build_lib():
+    # The LBM_ARITH_OPENCL kernels need OpenCL's FP32 division precision, a
+    # per-translation-unit flag: that unit is compiled on its own.
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    opencl_obj = os.path.join(OBJ_DIR, "lbm_opencl_arith.o")
+    _run([HIPCC, *HIP_FLAGS, "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-c", "-I", INCLUDE, "-I", CSRC,
+          os.path.join(CSRC, "lbm_opencl_arith.hip"), "-o", opencl_obj])
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
-          os.path.join(CSRC, "lbm_capi.hip"), os.path.join(CSRC, "lbm_adjoint.hip"), "-o", LIB])
+          os.path.join(CSRC, "lbm_capi.hip"), os.path.join(CSRC, "lbm_adjoint.hip"), "-x", "none", opencl_obj, "-o", LIB])
 
 
 def build_membw():

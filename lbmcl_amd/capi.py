@@ -24,6 +24,8 @@ LBM_ERR_STATE = -5
 LBM_F32, LBM_F64 = 0, 1
 LBM_STORE_NATIVE, LBM_STORE_FP16_SHIFT, LBM_STORE_BF16_SHIFT = 0, 1, 2
 LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_VEC8, LBM_KERNEL_TILE = 0, 1, 2, 3, 4, 5
+# lbm_config.optimize: arithmetic form of the collision
+LBM_ARITH_EXACT, LBM_ARITH_FUSED, LBM_ARITH_OPENCL = 0, 1, 2
 
 EVENT_NAMES = {0: "initialize", 1: "compute", 2: "read_map", 3: "read_f", 4: "read_rho", 5: "read_u",
                6: "diagnostics"}
@@ -193,7 +195,8 @@ class Context:
         cfg.viscosity = viscosity
         cfg.velocity = velocity
         cfg.lws[0], cfg.lws[1], cfg.lws[2] = lws
-        cfg.optimize = int(bool(optimize))
+        # LBM_ARITH_*; False and True select the exact and the fused (-o) form
+        cfg.optimize = int(optimize)
         cfg.kernel = kernel
         cfg.profile = int(bool(profile))
         cfg.xcd_remap = int(bool(xcd_remap))

@@ -72,6 +72,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         return fail(LBM_ERR_INVALID, "unknown storage");
     if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
         return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
+    if (k.optimize != LBM_ARITH_EXACT && k.optimize != LBM_ARITH_FUSED && k.optimize != LBM_ARITH_OPENCL)
+        return fail(LBM_ERR_INVALID, "optimize must be LBM_ARITH_EXACT, LBM_ARITH_FUSED or LBM_ARITH_OPENCL");
+    if (k.optimize == LBM_ARITH_OPENCL && k.storage != LBM_STORE_NATIVE)
+        return fail(LBM_ERR_INVALID, "LBM_ARITH_OPENCL needs native storage (16-bit storage uses the fused form)");
     if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
         k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
         return fail(LBM_ERR_INVALID, "unknown kernel selection");

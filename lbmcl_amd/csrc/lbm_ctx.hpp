@@ -341,9 +341,22 @@ inline int64_t compute_blocks(const lbm_ctx* c) {
     return (c->n3 + b - 1) / b;
 }
 
-// The fused collision is used for 16-bit storage always and for native
-// storage when fast math (-o) is requested.
-inline bool use_fast(const lbm_ctx* c) { return c->cfg.storage != LBM_STORE_NATIVE || c->cfg.optimize != 0; }
+// Arithmetic form of the collision: the fused form for 16-bit storage always
+// and for native storage with -o (LBM_ARITH_FUSED); LBM_ARITH_OPENCL evaluates
+// kernels.cl's expressions the way ROCm's OpenCL compiler does.
+inline int arith_of(const lbm_ctx* c) {
+    if (c->cfg.storage != LBM_STORE_NATIVE) return lbm::AR_FUSED;
+    if (c->cfg.optimize == LBM_ARITH_FUSED) return lbm::AR_FUSED;
+    if (c->cfg.optimize == LBM_ARITH_OPENCL) return lbm::AR_OPENCL;
+    return lbm::AR_EXACT;
+}
+
+// Compute launches in the OpenCL arithmetic form, defined in
+// lbm_opencl_arith.hip (built with OpenCL's FP32 division precision).
+int launch_compute_opencl_f32(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+                              hipEvent_t e1);
+int launch_compute_opencl_f64(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+                              hipEvent_t e1);
 
 // Launch on the context's stream; with events, through hipExtLaunchKernel,
 // which stamps the kernel's own start and end into them (no separate marker
@@ -356,7 +369,7 @@ inline void launch_on(lbm_ctx* c, K kernel, dim3 grid, dim3 block, hipEvent_t st
         hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, a);
 }
 
-template <class T, int KIND, bool FAST>
+template <class T, int KIND, int ARITH>
 inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                                hipEvent_t e1) {
     lbm::StepArgs<T> a{};
@@ -381,30 +394,35 @@ inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int updat
     {
         // lid-layer initial populations, same expression as the initialize kernel
         const T U = T(c->U);
-        const T u2 = ((U * U) + (T(0) * T(0))) + (T(0) * T(0));
-        for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq<T>(i, T(1), U, T(0), T(0), u2);
+        if constexpr (ARITH == lbm::AR_OPENCL) {
+            const T u2 = lbm::u2_contracted<T>(U, T(0), T(0));
+            for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq_contracted<T>(i, T(1), U, T(0), T(0), u2);
+        } else {
+            const T u2 = ((U * U) + (T(0) * T(0))) + (T(0) * T(0));
+            for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq<T>(i, T(1), U, T(0), T(0), u2);
+        }
     }
     const KernelKind kk = kernel_kind(c);
     if (kk == K_TILE) {
         const dim3 grid(unsigned(c->n / tile_b(c)), unsigned(c->n / tile_ty(c)), unsigned(c->n / tile_tz(c)));
         const dim3 block(unsigned(tile_b(c) * tile_ty(c) * tile_tz(c)));
-        launch_on(c, lbm::lbm_collide_stream_tile<T, KIND, FAST>, grid, block, e0, e1, a);
+        launch_on(c, lbm::lbm_collide_stream_tile<T, KIND, ARITH>, grid, block, e0, e1, a);
     } else if (kk != K_GENERIC) {
         a.gx = int32_t(row_gx(c));
         a.log2gx = ilog2(a.gx);
         const int64_t nb = compute_blocks(c);
         const unsigned bx = unsigned(row_block(c));
         if (kk == K_VEC)
-            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
+            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
         else if (kk == K_VEC8)
-            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
+            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
         else
-            launch_on(c, lbm::lbm_collide_stream_row<T, KIND, FAST>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
+            launch_on(c, lbm::lbm_collide_stream_row<T, KIND, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
     } else {
         const int b = generic_block(c);
         const int64_t nb = (c->n3 + b - 1) / b;
         a.gx = 0;
-        launch_on(c, lbm::lbm_collide_stream_generic<T, KIND, FAST>, dim3(unsigned(nb)), dim3(b), e0, e1, a);
+        launch_on(c, lbm::lbm_collide_stream_generic<T, KIND, ARITH>, dim3(unsigned(nb)), dim3(b), e0, e1, a);
     }
     LBM_HIP_CHECK(hipGetLastError(), "compute launch");
     return LBM_OK;
@@ -415,10 +433,16 @@ template <class T, int KIND>
 inline int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr) {
     if constexpr (KIND != lbm::ST_NATIVE) {
-        return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag, e0, e1);
+        return launch_compute_impl<T, KIND, lbm::AR_FUSED>(c, src, dst, update_macro, diag, e0, e1);
     } else {
-        if (use_fast(c)) return launch_compute_impl<T, KIND, true>(c, src, dst, update_macro, diag, e0, e1);
-        return launch_compute_impl<T, KIND, false>(c, src, dst, update_macro, diag, e0, e1);
+        const int ar = arith_of(c);
+        if (ar == lbm::AR_FUSED)
+            return launch_compute_impl<T, KIND, lbm::AR_FUSED>(c, src, dst, update_macro, diag, e0, e1);
+        if (ar == lbm::AR_OPENCL) {
+            if constexpr (sizeof(T) == 4) return launch_compute_opencl_f32(c, src, dst, update_macro, diag, e0, e1);
+            else return launch_compute_opencl_f64(c, src, dst, update_macro, diag, e0, e1);
+        }
+        return launch_compute_impl<T, KIND, lbm::AR_EXACT>(c, src, dst, update_macro, diag, e0, e1);
     }
 }
 
@@ -436,6 +460,7 @@ inline int launch_initialize(lbm_ctx* c) {
     a.U = T(c->U);
     a.scale = float(c->scale);
     a.inv_scale = float(1.0 / c->scale);
+    a.contracted = arith_of(c) == lbm::AR_OPENCL;
     const int64_t nb = (c->n3 + 255) / 256;
     hipLaunchKernelGGL((lbm::lbm_initialize_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
     LBM_HIP_CHECK(hipGetLastError(), "initialize launch");

@@ -143,10 +143,39 @@ template <class T> __host__ __device__ __forceinline__ T feq(int i, T rho, T ux,
     return (rho * w_of<T>(i)) * (((T(1) + (T(3) * eu)) + ((T(4.5) * eu) * eu)) - (T(1.5) * u2));
 }
 
-// Local part of `compute` (kernels.cl:343-418) for a non-wall cell of type t.
-// On entry f holds the gathered populations, on return the ones to store.
+// The same expressions as clang compiles OpenCL C, whose default is
+// FP_CONTRACT ON: within one expression x + a*b and x - a*b become fma(a, b, x)
+// and fma(-a, b, x), the left operand's product taken when both operands are
+// products.  In kernels.cl that touches u2, the equilibrium and the BGK
+// update; e_i . u multiplies by 0 and +-1 only, so it is exact in both forms.
+template <class T> __host__ __device__ __forceinline__ T fma_any(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+// ((ux ux) + (uy uy)) + (uz uz) (kernels.cl:390)
+template <class T> __host__ __device__ __forceinline__ T u2_contracted(T ux, T uy, T uz) {
+    return fma_any<T>(uz, uz, fma_any<T>(ux, ux, uy * uy));
+}
+// (rho w_i) (((1 + 3 eu) + 4.5 eu eu) - 1.5 u2) (kernels.cl:397, 416)
+template <class T> __host__ __device__ __forceinline__ T feq_contracted(int i, T rho, T ux, T uy, T uz, T u2) {
+    const T eu = edot<T>(i, ux, uy, uz);
+    const T a = fma_any<T>(T(3), eu, T(1));
+    const T b = fma_any<T>(T(4.5) * eu, eu, a);
+    return (rho * w_of<T>(i)) * fma_any<T>(-T(1.5), u2, b);
+}
+
+// Arithmetic forms of the local step: the reference's expressions evaluated
+// operation by operation (IEEE, no contraction; the CPU oracle bit for bit),
+// the fused collision (-o, 16-bit storage), and the reference's expressions as
+// ROCm's OpenCL compiler evaluates them (contracted as above, FP32 division to
+// OpenCL's 2.5 ulp; those kernels live in lbm_opencl_arith.hip, the one
+// translation unit built with that division precision).
+enum Arith { AR_EXACT = 0, AR_FUSED = 1, AR_OPENCL = 2 };
+
+// Lid fix-up and macroscopic values of a non-wall cell (kernels.cl:343-379):
+// on return f holds the fixed-up populations, rho and u the moments.
 template <class T>
-__device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+__device__ __forceinline__ void moments(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
                                         const StepConsts<T>& k) {
     const bool moving = is_moving(t);
     if (moving) {  // kernels.cl:343-349
@@ -171,8 +200,30 @@ __device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& 
         uy = jy / rho;
         uz = jz / rho;
     }
+}
+
+// Half-way bounce-back: swap opposite populations (kernels.cl:400-408).
+template <class T> __device__ __forceinline__ void bounce_back(T (&f)[Q]) {
+    T tmp;
+    tmp = f[1]; f[1] = f[3]; f[3] = tmp;
+    tmp = f[2]; f[2] = f[4]; f[4] = tmp;
+    tmp = f[5]; f[5] = f[6]; f[6] = tmp;
+    tmp = f[7]; f[7] = f[9]; f[9] = tmp;
+    tmp = f[8]; f[8] = f[10]; f[10] = tmp;
+    tmp = f[11]; f[11] = f[17]; f[17] = tmp;
+    tmp = f[12]; f[12] = f[18]; f[18] = tmp;
+    tmp = f[13]; f[13] = f[15]; f[15] = tmp;
+    tmp = f[14]; f[14] = f[16]; f[16] = tmp;
+}
+
+// Local part of `compute` (kernels.cl:343-418) for a non-wall cell of type t.
+// On entry f holds the gathered populations, on return the ones to store.
+template <class T>
+__device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+                                        const StepConsts<T>& k) {
+    moments<T>(t, f, rho, ux, uy, uz, k);
     const T u2 = ((ux * ux) + (uy * uy)) + (uz * uz);  // kernels.cl:390
-    if (moving) {
+    if (is_moving(t)) {
         // kernels.cl:393-398 sets f_i = feq_i; the BGK step that follows
         // (kernels.cl:412-418) then adds inv_tau * (feq_i - f_i) = 0 exactly.
 #pragma unroll
@@ -183,19 +234,32 @@ __device__ __forceinline__ void collide(int32_t t, T (&f)[Q], T& rho, T& ux, T& 
             const T e = feq<T>(i, rho, ux, uy, uz, u2);
             f[i] = f[i] + k.inv_tau * (e - f[i]);
         }
-    } else if (is_bounceback(t)) {  // kernels.cl:400-408
-        T tmp;
-        tmp = f[1]; f[1] = f[3]; f[3] = tmp;
-        tmp = f[2]; f[2] = f[4]; f[4] = tmp;
-        tmp = f[5]; f[5] = f[6]; f[6] = tmp;
-        tmp = f[7]; f[7] = f[9]; f[9] = tmp;
-        tmp = f[8]; f[8] = f[10]; f[10] = tmp;
-        tmp = f[11]; f[11] = f[17]; f[17] = tmp;
-        tmp = f[12]; f[12] = f[18]; f[18] = tmp;
-        tmp = f[13]; f[13] = f[15]; f[15] = tmp;
-        tmp = f[14]; f[14] = f[16]; f[16] = tmp;
+    } else if (is_bounceback(t)) {
+        bounce_back<T>(f);
     }
     // CORNER: pure propagation (kernels.cl:254-260 with no BC and no collision).
+}
+
+// collide() with the contractions of ROCm's OpenCL compiler (see
+// feq_contracted); the divisions round as the translation unit compiles them.
+template <class T>
+__device__ __forceinline__ void collide_opencl(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
+                                               const StepConsts<T>& k) {
+    moments<T>(t, f, rho, ux, uy, uz, k);
+    const T u2 = u2_contracted<T>(ux, uy, uz);
+    if (is_moving(t)) {
+        // f_i = feq_i, then f_i + inv_tau (feq_i - f_i) = fma(inv_tau, 0, feq_i) = feq_i
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = feq_contracted<T>(i, rho, ux, uy, uz, u2);
+    } else if (t == FLUID) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T e = feq_contracted<T>(i, rho, ux, uy, uz, u2);
+            f[i] = fma_any<T>(k.inv_tau, e - f[i], f[i]);  // kernels.cl:270-273
+        }
+    } else if (is_bounceback(t)) {
+        bounce_back<T>(f);
+    }
 }
 
 // Fused form of the same local step, used by the 16-bit storage forms and by
@@ -215,41 +279,9 @@ template <class T>
 __device__ __forceinline__ void collide_fast(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
                                              const StepConsts<T>& k) {
     const bool moving = is_moving(t);
-    if (moving) {
-        f[5] = f[6];
-        f[11] = f[17];
-        f[12] = f[18];
-        f[13] = f[15];
-        f[14] = f[16];
-    }
-    rho = f[0];
-#pragma unroll
-    for (int i = 1; i < Q; ++i) rho = rho + f[i];
-    if (moving) {
-        ux = k.U;
-        uy = T(0);
-        uz = T(0);
-    } else {
-        const T jx = ((((f[1] + f[7]) + f[10]) + f[11]) + f[15]) - ((((f[3] + f[8]) + f[9]) + f[13]) + f[17]);
-        const T jy = ((((f[2] + f[7]) + f[8]) + f[12]) + f[16]) - ((((f[4] + f[9]) + f[10]) + f[14]) + f[18]);
-        const T jz = ((((f[6] + f[15]) + f[16]) + f[17]) + f[18]) - ((((f[5] + f[11]) + f[12]) + f[13]) + f[14]);
-        ux = jx / rho;
-        uy = jy / rho;
-        uz = jz / rho;
-    }
+    moments<T>(t, f, rho, ux, uy, uz, k);
     if (!moving && t != FLUID) {
-        if (is_bounceback(t)) {
-            T tmp;
-            tmp = f[1]; f[1] = f[3]; f[3] = tmp;
-            tmp = f[2]; f[2] = f[4]; f[4] = tmp;
-            tmp = f[5]; f[5] = f[6]; f[6] = tmp;
-            tmp = f[7]; f[7] = f[9]; f[9] = tmp;
-            tmp = f[8]; f[8] = f[10]; f[10] = tmp;
-            tmp = f[11]; f[11] = f[17]; f[17] = tmp;
-            tmp = f[12]; f[12] = f[18]; f[18] = tmp;
-            tmp = f[13]; f[13] = f[15]; f[15] = tmp;
-            tmp = f[14]; f[14] = f[16]; f[16] = tmp;
-        }
+        if (is_bounceback(t)) bounce_back<T>(f);
         return;
     }
     const T u2 = fmaT<T>(uz, uz, fmaT<T>(uy, uy, ux * ux));
@@ -283,10 +315,11 @@ __device__ __forceinline__ void add_force(int32_t t, T (&f)[Q], T gx, T gy, T gz
     for (int i = 0; i < Q; ++i) f[i] = f[i] + (T(3) * w_of<T>(i)) * edot<T>(i, gx, gy, gz);
 }
 
-template <class T, bool FAST>
+template <class T, int ARITH>
 __device__ __forceinline__ void collide_any(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
                                             const StepConsts<T>& k) {
-    if constexpr (FAST) collide_fast<T>(t, f, rho, ux, uy, uz, k);
+    if constexpr (ARITH == AR_FUSED) collide_fast<T>(t, f, rho, ux, uy, uz, k);
+    else if constexpr (ARITH == AR_OPENCL) collide_opencl<T>(t, f, rho, ux, uy, uz, k);
     else collide<T>(t, f, rho, ux, uy, uz, k);
 }
 
@@ -384,7 +417,7 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb, i
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <class T, int KIND, bool FAST>
+template <class T, int KIND, int ARITH>
 __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
     using C = Codec<T, KIND>;
     using S = typename C::S;
@@ -423,7 +456,7 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
             const uint32_t off = ex == 1 ? v_m : (ex == -1 ? v_p : v_0);
             f[i] = codec.load(src + base + off, w_of<T>(i));
         }
-        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         if (a.force) {
             const int64_t c = int64_t(x) + int64_t(n) * r;
             add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
@@ -508,7 +541,7 @@ template <class T, int KIND, int LB> struct VecWaves {
     static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? 4 : 3);
 };
 
-template <class T, int KIND, int LB, bool FAST>
+template <class T, int KIND, int LB, int ARITH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VecWaves<T, KIND, LB>::value)))
 void lbm_collide_stream_vec(StepArgs<T> a) {
     using C = Codec<T, KIND>;
@@ -621,7 +654,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] = codec.decode(g[i][j], w_of<T>(i));
         T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
-        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
+        collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
         if (a.force) {
             const int64_t c = int64_t(x0 + j) + int64_t(n) * r;
             add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
@@ -748,7 +781,7 @@ template <class T, int KIND> struct TileThreads {
     static constexpr int value = TS::B * TS::TY * TS::TZ;
 };
 
-template <class T, int KIND, bool FAST>
+template <class T, int KIND, int ARITH>
 __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_stream_tile(StepArgs<T> a) {
     using C = Codec<T, KIND>;
     using S = typename C::S;
@@ -845,7 +878,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
             else v = span[r][csoa(xs, i)];
             f[i] = codec.decode(v, w_of<T>(i));
         }
-        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         const int64_t c = int64_t(x) + int64_t(n) * (int64_t(y) + int64_t(n) * z);
         if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
         if (a.update_macro && is_store_macro(t)) {
@@ -896,7 +929,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
 // Generic kernel: any N, any stride, any block size.  One thread per cell of a
 // 1-D grid over all N^3 cells, full 64-bit slot arithmetic per population.
 // ---------------------------------------------------------------------------
-template <class T, int KIND, bool FAST>
+template <class T, int KIND, int ARITH>
 __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a) {
     using C = Codec<T, KIND>;
     using S = typename C::S;
@@ -919,7 +952,7 @@ __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a
             const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
             f[i] = codec.load(src + slot(s, i, a.stride), w_of<T>(i));
         }
-        collide_any<T, FAST>(t, f, rho, ux, uy, uz, a.k);
+        collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) codec.store(dst + slot(c, i, a.stride), w_of<T>(i), f[i]);
@@ -958,6 +991,7 @@ template <class T> struct InitArgs {
     int32_t log2n;
     T U;
     float scale, inv_scale;
+    int32_t contracted;  // nonzero: the equilibrium as ROCm's OpenCL compiler evaluates it
 };
 
 template <class T, int KIND>
@@ -989,8 +1023,12 @@ __global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
             if (!is_wall(td)) {
                 // kernels.cl:292-308 with rho = 1, u = (U or 0, 0, 0)
                 const T ux = is_moving_init(td) ? a.U : T(0);
-                const T u2 = ((ux * ux) + (T(0) * T(0))) + (T(0) * T(0));
-                v = feq<T>(i, T(1), ux, T(0), T(0), u2);
+                if (a.contracted) {
+                    v = feq_contracted<T>(i, T(1), ux, T(0), T(0), u2_contracted<T>(ux, T(0), T(0)));
+                } else {
+                    const T u2 = ((ux * ux) + (T(0) * T(0))) + (T(0) * T(0));
+                    v = feq<T>(i, T(1), ux, T(0), T(0), u2);
+                }
             }
         }
         const int64_t s = slot(c, i, a.stride);
