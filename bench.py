@@ -14,9 +14,10 @@ Printed: one JSON line with MLUPS (the reference's definition,
 (dim-2)^3 * iterations / time, lbmcl.hpp:601-620), the HBM roofline of the
 compute kernel (152 B per lattice update in FP32; average kernel duration from
 HIP events recorded on the kernel's stream around every launch of the timed
-region), the same figures for the 16-bit shifted-storage forms and FP64, a copy
-kernel's sustained bandwidth for context, and the CPU oracle timed on a bounded
-sample of the same workload (cpu_baseline).
+region), the same figures for the 16-bit shifted-storage forms, FP64, the fused
+collision (-o) and the reference's own OpenCL rounding (LBM_ARITH_OPENCL), a
+copy kernel's sustained bandwidth for context, and the CPU oracle timed on a
+bounded sample of the same workload (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -40,11 +41,12 @@ def log(msg):
 
 
 def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True,
-                fused=False):
+                arith=0):
+    """`arith`: lbm_config.optimize (0 exact, 1 fused, 2 the reference's OpenCL rounding)."""
     from lbmcl_amd import capi
     stride = stride or dim ** 3
     with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=True,
-                      xcd_remap=xcd_remap, optimize=fused) as g:
+                      xcd_remap=xcd_remap, optimize=arith) as g:
         g.initialize()
         it = 1
         if warmup:
@@ -148,12 +150,13 @@ def main():
 
     variants = {}
     if args.variant_steps > 0:
-        for prec, st, fused in (("f32", "fp16", True), ("f32", "bf16", True), ("f64", "native", False),
-                                ("f32", "native", True)):
-            name = f"{prec}_{st}" + ("_fused" if st == "native" and fused else "")
+        for name, prec, st, arith in (("f32_fp16", "f32", "fp16", 1), ("f32_bf16", "f32", "bf16", 1),
+                                      ("f64_native", "f64", "native", 0), ("f32_native_fused", "f32", "native", 1),
+                                      ("f32_native_opencl_arith", "f32", "native", 2),
+                                      ("f64_native_opencl_arith", "f64", "native", 2)):
             log(f"[bench] variant {name}")
             r = run_lattice(dim, args.variant_steps, max(5, args.warmup // 4), prec, st, stride, args.kernel,
-                            not args.no_remap, fused)
+                            not args.no_remap, arith)
             variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
                                                 "bytes_per_update", "steps")}
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")

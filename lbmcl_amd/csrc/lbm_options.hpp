@@ -38,6 +38,7 @@ struct lbm_options {
     bool dump_map = false;
     bool dump_f = false;
     int storage = LBM_STORE_NATIVE;
+    int arith = -1;  // --arith: LBM_ARITH_EXACT or LBM_ARITH_OPENCL; -1: from -o
 
     [[noreturn]] void print_help() {
         std::cout << "-P  --platform            Use the specified platform                     \n"
@@ -56,6 +57,8 @@ struct lbm_options {
                      "-m  --dump_map            Dump the lattice map                           \n"
                      "-f  --dump_f              Dump the lattice \"f\" for each iteration      \n"
                      "    --storage             Population storage: native, fp16 or bf16       \n"
+                     "    --arith               Collision arithmetic: exact, or opencl for the \n"
+                     "                          reference's own rounding on this GPU           \n"
                      "-h  --help                Show this help message and exit                \n";
         std::exit(1);
     }
@@ -89,6 +92,7 @@ struct lbm_options {
             {"dump_map", no_argument, nullptr, 'm'},
             {"dump_f", no_argument, nullptr, 'f'},
             {"storage", required_argument, nullptr, 'S'},
+            {"arith", required_argument, nullptr, 'A'},
             {"help", no_argument, nullptr, 'h'},
             {nullptr, no_argument, nullptr, 0},
         };
@@ -168,11 +172,24 @@ struct lbm_options {
                     else if (!std::strcmp(arg, "bf16")) storage = LBM_STORE_BF16_SHIFT;
                     else { std::cerr << "Please enter a valid storage (native, fp16, bf16)" << std::endl; std::exit(1); }
                     break;
+                case 'A':
+                    if (!std::strcmp(arg, "exact")) arith = LBM_ARITH_EXACT;
+                    else if (!std::strcmp(arg, "opencl")) arith = LBM_ARITH_OPENCL;
+                    else { std::cerr << "Please enter a valid arithmetic (exact, opencl)" << std::endl; std::exit(1); }
+                    break;
                 case 'h':
                 case '?':
                 default:
                     print_help();
             }
+        }
+        if (arith >= 0 && optimize) {
+            std::cerr << "-o selects the fused arithmetic; drop --arith or -o" << std::endl;
+            std::exit(1);
+        }
+        if (arith == LBM_ARITH_OPENCL && storage != LBM_STORE_NATIVE) {
+            std::cerr << "--arith opencl needs native storage" << std::endl;
+            std::exit(1);
         }
         if (storage != LBM_STORE_NATIVE && use_double) {
             std::cerr << "16-bit shifted storage computes in single precision; drop -F" << std::endl;

@@ -61,6 +61,7 @@ class LBMCL {
     bool dump_f;
     bool dump_data = false;
     int storage = LBM_STORE_NATIVE;
+    int arith = -1;  // LBM_ARITH_*; -1: from optimize
 
     int device_id = -1;
     std::string device_name;
@@ -212,6 +213,12 @@ class LBMCL {
     // reference.
     void setStorage(int s) { storage = s; }
 
+    // Collision arithmetic (LBM_ARITH_*); call before setupSimulation.  By
+    // default the constructor's `optimize` selects LBM_ARITH_FUSED or
+    // LBM_ARITH_EXACT; LBM_ARITH_OPENCL reproduces the reference's own
+    // rounding on this GPU.
+    void setArithmetic(int a) { arith = a; }
+
     // Select the device and allocate all device memory (lbmcl.hpp:392-482).
     // platformID is accepted for interface compatibility: HIP exposes one
     // platform, and any id other than 0 or -1 is reported and treated as 0.
@@ -233,7 +240,7 @@ class LBMCL {
         cfg.lws[0] = int32_t(lws[0]);
         cfg.lws[1] = int32_t(lws[1]);
         cfg.lws[2] = int32_t(lws[2]);
-        cfg.optimize = optimize ? 1 : 0;
+        cfg.optimize = arith >= 0 ? arith : (optimize ? LBM_ARITH_FUSED : LBM_ARITH_EXACT);
         cfg.kernel = LBM_KERNEL_AUTO;
         cfg.profile = 1;
         cfg.xcd_remap = 1;

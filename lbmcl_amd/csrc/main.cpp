@@ -12,6 +12,7 @@ void performSimulation(const lbm_options& opts) {
                    opts.lwx, opts.lwy, opts.lwz, opts.stride, opts.optimize, opts.dump_path, opts.dump_map,
                    opts.dump_f);
     lbmcl.setStorage(opts.storage);
+    if (opts.arith >= 0) lbmcl.setArithmetic(opts.arith);
     lbmcl.setupSimulation(opts.platformID, opts.deviceID);
     lbmcl.printConfiguration();
     lbmcl.performSimulation();

@@ -94,7 +94,7 @@ def test_cli_help(lbmcl_bin):
     assert r.returncode == 1
     for opt in ("--platform", "--dim", "--viscosity", "--velocity", "--iterations", "--every", "--work_group_size",
                 "--stride", "--use_double", "--optimize", "--vtk_path", "--dump_path", "--dump_map", "--dump_f",
-                "--storage"):
+                "--storage", "--arith"):
         assert opt in r.stdout
 
 
@@ -108,6 +108,9 @@ def test_cli_help(lbmcl_bin):
     (["-D", "-1"], "Please enter a valid device"),
     (["--storage", "fp8"], "Please enter a valid storage"),
     (["-F", "--storage", "fp16"], "single precision"),
+    (["--arith", "fast"], "Please enter a valid arithmetic"),
+    (["-o", "--arith", "opencl"], "drop --arith or -o"),
+    (["--arith", "opencl", "--storage", "bf16"], "needs native storage"),
 ])
 def test_cli_rejects_bad_values(lbmcl_bin, args, msg):
     r = run_cli(lbmcl_bin, *args)

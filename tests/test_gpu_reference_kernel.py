@@ -24,14 +24,17 @@ work-group sizes at 256^3 (the largest lattice its int indexing allows) and
 requires the HIP path to be faster than the best of them.
 """
 import os
+import subprocess
 
 import numpy as np
 import pytest
 
 import oracle as O
+from lbmcl_amd import build as native_build
 from lbmcl_amd import capi
+from lbmcl_amd.vti import read_vti
 from oracle import ref_build
-from tests.util import max_rel
+from tests.util import max_rel, window, window_v
 
 pytestmark = pytest.mark.gpu
 
@@ -194,3 +197,20 @@ def test_faster_than_reference_kernels(precision):
           f"arithmetic {ours_ocl:.0f}; best reference {best_ref[0]:.0f} ({best_ref[1]}); ratio "
           f"{ours / best_ref[0]:.2f}", flush=True)
     assert min(ours, ours_fast, ours_ocl) > best_ref[0]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_cli_opencl_arith_vti_is_the_reference_output(tmp_path, precision):
+    """`lbmcl --arith opencl` writes the VTI the reference's kernels produce on
+    this GPU: the make test32 run (32^3, 500 iterations, stride 32), last VTI."""
+    n, iterations = 32, 500
+    args = ["-P0", "-D0", "-d", str(n), "-i", str(iterations), "-e", str(iterations), "-s", "32", "-w", "32,8,1",
+            "--arith", "opencl", "-v", str(tmp_path)] + (["-F"] if precision == "f64" else [])
+    print("lbmcl", " ".join(args), flush=True)
+    r = subprocess.run([native_build.CLI, *args], capture_output=True, text=True, timeout=300,
+                       stdin=subprocess.DEVNULL)
+    assert r.returncode == 0, r.stderr
+    vt = read_vti(str(tmp_path / f"lbmcl.{iterations}.vti"))
+    ref = O.ref_run(precision, n, 32, (32, 8, 1), iterations, out_prefix=str(tmp_path / "ref"))
+    _bits_equal(vt["rho"], window(ref["rho"], n), f"lbmcl --arith opencl {precision} VTI rho")
+    _bits_equal(vt["v"], window_v(ref["u"], n), f"lbmcl --arith opencl {precision} VTI v")
