@@ -47,7 +47,7 @@ for _p in ("f32", "f64"):
     CONFIGS += [(_p, 8, 8, 8, False), (_p, 32, 32, 32, False), (_p, 64, 64, 64, False), (_p, 64, 1, 64, False),
                 (_p, 128, 64, 64, False)]
     CONFIGS += [(_p, 256, s, 64, False) for s in (1, 64, 4096, 256 ** 3)]
-    CONFIGS += [(_p, 128, 128 ** 3, 64, False)]
+    CONFIGS += [(_p, 128, s, 64, False) for s in (1, 4096, 128 ** 3)]
 CONFIGS += [("f32", 32, 32, 32, True), ("f32", 128, 64, 64, True), ("f32", 256, 64, 64, True),
             ("f32", 256, 4096, 64, True)]
 
