@@ -176,7 +176,7 @@ def _hip_mlups(precision, n, stride, steps, warmup, arith=capi.LBM_ARITH_EXACT):
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_faster_than_reference_kernels(precision, n):
     """The reference's compute kernel over strides 1, 64, 4096, N^3 and work-groups
-    (64,1,1), (256,1,1) -- plus its -o build for FP32 at 256^3 -- against the HIP
+    of 64 and 256 work-items -- plus its -o build for FP32 at 256^3 -- against the HIP
     path, kernel MLUPS from profiling events (the reference's kernelsMLUPS,
     lbmcl.hpp:606-619).  256^3 (the largest lattice the reference indexes): 200
     iterations after 20, HIP at stride 4096.  128^3: the 1000-iteration
@@ -189,7 +189,8 @@ def test_faster_than_reference_kernels(precision, n):
         for stride in (1, 64, 4096, n ** 3):
             if not os.path.exists(ref_build.co_path(precision, n, stride, fast)):
                 continue
-            for lws in ((64, 1, 1), (256, 1, 1)):
+            # 64 and 256 work-items (256 as (128, 2, 1) at 128^3: x must divide dim)
+            for lws in ((64, 1, 1), (min(n, 256), 256 // min(n, 256), 1)):
                 r = O.ref_run(precision, n, stride, lws, warmup + steps, every=0, skip=warmup, fast=fast)
                 rows.append((r["mlups"], f"reference kernels.cl{' -o' if fast else ''} stride {stride} lws {lws}"))
                 print(f"{n}^3 {precision} {rows[-1][1]}: {r['mlups']:.0f} MLUPS", flush=True)
