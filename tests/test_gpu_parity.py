@@ -3,9 +3,9 @@
 The HIP path and the oracle evaluate the same expressions in the same order
 without contraction, so the expected agreement is bit for bit: rho, u, the full
 CSoA lattice in the reference's push view, and the raw stored state.  The
-north-star tolerances (1e-5 relative in FP32, 1e-12 in FP64) are asserted as
-well, so a failure message distinguishes "bits differ" from "out of
-tolerance".
+tolerances of a same-precision comparison (1e-5 relative in FP32, 1e-12 in
+FP64) are asserted as well, so a failure message distinguishes "bits differ"
+from "out of tolerance".
 """
 import os
 
