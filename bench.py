@@ -179,7 +179,7 @@ def main():
 
     wet = (dim - 2) ** 3
     out = {
-        "metric": "MLUPS (D3Q19 lid-driven cavity, 512^3, FP32)",
+        "metric": f"MLUPS (D3Q19 lid-driven cavity, {dim}^3, FP32)",
         "value": round(main_run["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,

@@ -279,6 +279,30 @@ def test_full_size_512_f32():
         assert nan_equal(ug, o.u), mismatch_count(ug, o.u)
 
 
+def test_full_size_1024_f32_vector_matches_generic():
+    """1024^3 FP32 -- 163 GB of lattices, element offsets beyond 2^32 (19 N^3 =
+    2.0e10), a size the reference's int index cannot address and the CPU oracle
+    is too large for: the 8-byte-lane vector kernel (two blocks per row) and the
+    one-thread-per-cell generic kernel, both bit-identical to the oracle at
+    smaller sizes, agree bit for bit on rho and u after two steps, and on the
+    FP64 mass and energy to their summation order."""
+    n = 1024
+    out = {}
+    for kernel in (capi.LBM_KERNEL_VEC8, capi.LBM_KERNEL_GENERIC):
+        print(f"1024^3 FP32 stride 4096 kernel {kernel}", flush=True)
+        with capi.Context(n, stride=4096, precision="f32", kernel=kernel) as g:
+            g.initialize()
+            g.compute(1)
+            g.compute(2, update_macro=True, diagnostics=True)
+            out[kernel] = (g.read_rho(), g.read_u(), g.read_diagnostics())
+    (rv, uv, dv), (rg, ug, dg) = out[capi.LBM_KERNEL_VEC8], out[capi.LBM_KERNEL_GENERIC]
+    assert nan_equal(rv, rg), mismatch_count(rv, rg)
+    assert int(np.isfinite(rv).sum()) > (n - 4) ** 3
+    del rv, rg
+    assert nan_equal(uv, ug), mismatch_count(uv, ug)
+    assert abs(dv[0] - dg[0]) <= 1e-12 * abs(dg[0]) and abs(dv[1] - dg[1]) <= 1e-10 * abs(dg[1])
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_cavity_128_1000_steps(precision):
     """The 128^3 GPU configuration: stride 64, 1000 iterations, rho and u after
