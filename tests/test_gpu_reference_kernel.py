@@ -9,8 +9,8 @@ runtime with the host flow of lbmcl.hpp:392-521 (oracle/ref_runner.c).  This is
   compiler does (multiply-adds contracted as clang forms them, FP32 division
   to 2.5 ulp): rho, u and the full CSoA lattice that the next iteration reads
   are bit-identical to the reference's, for every kernel form.
-* LBM_ARITH_EXACT (the default, bit-identical to the CPU oracle) evaluates
-  every operation as written.  Against the reference it differs by FP32/FP64
+* LBM_ARITH_EXACT (the default, bit-identical to the CPU oracle, which the
+  cases up to 64^3 also run) evaluates every operation as written.  Against the reference it differs by FP32/FP64
   rounding only: within 1e-12 relative in FP64; in FP32 the difference grows
   with the number of steps, so it is bounded by the reference's own FP32
   rounding error -- measured against the reference run in FP64 -- instead.
@@ -129,6 +129,13 @@ def test_exact_arithmetic_against_reference_kernels(tmp_path, precision, n, stri
     print(f"reference kernels.cl vs HIP: {label}", flush=True)
     ref = O.ref_run(precision, n, stride, lws, iterations, out_prefix=str(tmp_path / "ref"))
     rho, u, f = _hip_run(precision, n, stride, iterations)
+    if n <= 64:
+        # the CPU oracle itself (bit-identical to the HIP exact form) against the reference
+        o = O.PullSim(n, stride=stride, precision=precision)
+        o.initialize()
+        for _ in range(iterations):
+            o.step(update_macro=True)
+        assert np.array_equal(o.rho, rho, equal_nan=True) and np.array_equal(o.u.reshape(3, -1), u, equal_nan=True)
     if precision == "f64":
         _compare(rho, ref["rho"], TOL["f64"], f"{label} rho")
         _compare(u, ref["u"], TOL["f64"], f"{label} u")
