@@ -129,6 +129,8 @@ def main(argv=None):
     ap.add_argument("--no-soa", action="store_true", help="skip the second sweep with stride dim^3")
     ap.add_argument("--repeats", type=int, default=10)
     ap.add_argument("--storage", choices=["native", "fp16", "bf16"], default="native")
+    ap.add_argument("--arith", choices=["exact", "opencl"], default=None,
+                    help="pass --arith to lbmcl (opencl: the reference's own rounding; excludes -o)")
     ap.add_argument("--no-optimize", action="store_true", help="omit -o (benchmark.sh always passes it)")
     ap.add_argument("--vtk-path", default="", help="pass -v (with --every > 0, files are written)")
     ap.add_argument("--cli", default=native_build.CLI)
@@ -149,8 +151,10 @@ def main(argv=None):
     def args_for(d, lws, s):
         r = ["-P", str(a.platform), "-D", str(a.device), "-d", str(d), "-n", a.viscosity, "-u", a.velocity,
              "-i", str(a.iterations), "-e", str(a.every), "-w", ",".join(map(str, lws)), "-s", str(s)]
-        if not a.no_optimize:
+        if not a.no_optimize and a.arith is None:
             r.append("-o")
+        if a.arith is not None:
+            r += ["--arith", a.arith]
         if a.precision == "double":
             r.append("-F")
         if a.storage != "native":

@@ -73,3 +73,19 @@ def test_driver_end_to_end(tmp_path):
     assert sum("-s 512" in a for a in args) == 4 * 3
     assert (out / "benchmark.csv").read_text() == "AMD Instinct MI355X;single;8;001,001,001;8;4;2;10;20\n"
     assert len((out / "wall.csv").read_text().splitlines()) == 36
+
+
+def test_driver_passes_arith(tmp_path):
+    """--arith opencl reaches lbmcl in place of -o."""
+    cli = tmp_path / "fake_lbmcl"
+    cli.write_text(textwrap.dedent("""\
+        #!/bin/sh
+        echo "$@" >> "$(dirname "$0")/args.txt"
+        echo "AMD Instinct MI355X;single;8;50;0;001,001,001;8;0;4;2;10;20" >&2
+        """))
+    cli.chmod(cli.stat().st_mode | stat.S_IEXEC)
+    rc = B.main(["--cli", str(cli), "--out", str(tmp_path / "b"), "--dims", "8", "--lws", "1", "--strides", "8",
+                 "--repeats", "3", "--no-soa", "--arith", "opencl"])
+    assert rc == 0
+    args = (tmp_path / "args.txt").read_text().splitlines()
+    assert len(args) == 3 and all(a.endswith("-s 8 --arith opencl") for a in args)
