@@ -63,9 +63,12 @@ def test_make_test8(tmp_path, golden_dir, optimize):
         assert np.array_equal(vt["v"], window_v(sim.u, 8), equal_nan=True), it
 
 
-def test_make_test32(tmp_path, golden_dir):
+@pytest.mark.parametrize("arith", [[], ["--arith", "opencl"]])
+def test_make_test32(tmp_path, golden_dir, arith):
+    """make test32; also in the reference's own rounding (--arith opencl), which
+    meets the same verify.py thresholds against the Sailfish fixtures."""
     r = run(["-P0", "-D0", "-d", "32", "-n", "0.0089", "-u", "0.05", "-i", "500", "-e", "20", "-w", "32,32,1", "-s",
-             "32", "-v", str(tmp_path)], tmp_path)
+             "32", "-v", str(tmp_path)] + arith, tmp_path)
     assert r.returncode == 0, r.stderr
     v = verify(500, 20, os.path.join(golden_dir, "target_results", "32"), str(tmp_path), (2e-12, 3e-14, 4e-6, 1.2e-6))
     assert v.returncode == 0, v.stdout + v.stderr
