@@ -303,6 +303,21 @@ def test_full_size_1024_f32_vector_matches_generic():
     assert abs(dv[0] - dg[0]) <= 1e-12 * abs(dg[0]) and abs(dv[1] - dg[1]) <= 1e-10 * abs(dg[1])
 
 
+def test_allocation_failure_is_an_error_and_leaves_the_device_usable():
+    """1024^3 FP64 needs two 163 GB lattices, more than the GPU's HBM: lbm_create
+    fails with LBM_ERR_MEMORY after releasing what it had allocated, and the next
+    context computes correctly."""
+    print("1024^3 FP64 (does not fit)", flush=True)
+    with pytest.raises(capi.LbmError) as e:
+        capi.Context(1024, stride=4096, precision="f64")
+    assert e.value.code == capi.LBM_ERR_MEMORY and "hipMalloc" in str(e.value)
+    g, o = _pair(16, 16, "f64")
+    with g:
+        g.compute(1, update_macro=True)
+        o.step(update_macro=True)
+        _assert_macro_equal(g, o, 16, "f64", "after a failed allocation")
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_cavity_128_1000_steps(precision):
     """The 128^3 GPU configuration: stride 64, 1000 iterations, rho and u after
