@@ -104,6 +104,20 @@ def test_opencl_arithmetic_is_bit_identical_to_reference_kernels(tmp_path, preci
     _bits_equal(f, ref["f"], f"{label} f")
 
 
+@pytest.mark.parametrize("stride", [1, 4096, 128 ** 3])
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_opencl_arithmetic_other_layouts(tmp_path, precision, stride):
+    """128^3 x 50 in AoS (stride 1, the tile kernel), CSoA 4096 and SoA (N^3)."""
+    n, iterations = 128, 50
+    label = f"OpenCL arithmetic {n}^3 {precision} stride {stride} x {iterations}"
+    print(f"reference kernels.cl vs HIP: {label}", flush=True)
+    ref = O.ref_run(precision, n, stride, (64, 1, 1), iterations, out_prefix=str(tmp_path / "ref"))
+    rho, u, f = _hip_run(precision, n, stride, iterations, arith=capi.LBM_ARITH_OPENCL)
+    _bits_equal(rho, ref["rho"], f"{label} rho")
+    _bits_equal(u, ref["u"], f"{label} u")
+    _bits_equal(f, ref["f"], f"{label} f")
+
+
 @pytest.mark.parametrize("precision,n,stride,kernel", [
     ("f32", 64, 64, capi.LBM_KERNEL_GENERIC), ("f32", 64, 64, capi.LBM_KERNEL_ROW),
     ("f32", 256, 64, capi.LBM_KERNEL_VEC), ("f32", 64, 1, capi.LBM_KERNEL_TILE),
