@@ -10,7 +10,9 @@ digits) and -DFP_SINGLE -cl-single-precision-constant or -DFP_DOUBLE, plus
 this clang stay warnings: FP64 adds -Wno-error=ignored-pragmas (clang does not
 know the cl_amd_fp64 extension that kernels.cl:25 enables) and -o adds
 -Wno-error=nan-infinity-disabled (NAN under finite-math-only).  OpenCL C 1.2
-and -O3 are the runtime's defaults.
+and -O3 are the runtime's defaults.  Like every GPU build here the code is
+compiled without packed-FP32 instructions, which changes instruction
+selection only (a packed FP32 operation rounds as its scalar form).
 No reference file is copied or changed; the outputs (code objects and the
 runner, oracle/ref_runner.c) go to oracle/_ref/ only.  The runner loads a code
 object with clCreateProgramWithBinary, so the GPU host needs the ROCm OpenCL
