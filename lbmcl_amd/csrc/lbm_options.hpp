@@ -2,8 +2,12 @@
 //
 // Same options, defaults and validation as the reference
 // (libs/lbm_options.hpp:13-196): -P -D -d -n -u -i -e -v -w -s -F -o -p -m -f -h.
-// One long-only addition: --storage native|fp16|bf16 selects the 16-bit
-// shifted population storage (FP32 arithmetic), which the reference lacks.
+// Two long-only additions: --storage native|fp16|bf16 selects the 16-bit
+// shifted population storage (FP32 arithmetic), which the reference lacks, and
+// --arith opencl|exact the collision arithmetic.  Without -o and with native
+// storage the default is opencl: the reference's kernels.cl evaluated as
+// ROCm's OpenCL compiler does, so the outputs are those of the reference on
+// the same GPU, bit for bit; exact evaluates each operation as written.
 #pragma once
 
 #include <getopt.h>
@@ -38,7 +42,7 @@ struct lbm_options {
     bool dump_map = false;
     bool dump_f = false;
     int storage = LBM_STORE_NATIVE;
-    int arith = -1;  // --arith: LBM_ARITH_EXACT or LBM_ARITH_OPENCL; -1: from -o
+    int arith = -1;  // LBM_ARITH_*; -1: LBM_ARITH_FUSED with -o or 16-bit storage
 
     [[noreturn]] void print_help() {
         std::cout << "-P  --platform            Use the specified platform                     \n"
@@ -57,8 +61,8 @@ struct lbm_options {
                      "-m  --dump_map            Dump the lattice map                           \n"
                      "-f  --dump_f              Dump the lattice \"f\" for each iteration      \n"
                      "    --storage             Population storage: native, fp16 or bf16       \n"
-                     "    --arith               Collision arithmetic: exact, or opencl for the \n"
-                     "                          reference's own rounding on this GPU           \n"
+                     "    --arith               Collision arithmetic: opencl (default, the     \n"
+                     "                          reference's own rounding on this GPU) or exact \n"
                      "-h  --help                Show this help message and exit                \n";
         std::exit(1);
     }
@@ -195,5 +199,6 @@ struct lbm_options {
             std::cerr << "16-bit shifted storage computes in single precision; drop -F" << std::endl;
             std::exit(1);
         }
+        if (arith < 0 && !optimize && storage == LBM_STORE_NATIVE) arith = LBM_ARITH_OPENCL;
     }
 };

@@ -43,8 +43,9 @@ def verify(iterations, every, target, pred, limits):
 
 @pytest.mark.parametrize("optimize", [False, True])
 def test_make_test8(tmp_path, golden_dir, optimize):
+    # compared value for value with the oracle: the exact arithmetic (or -o)
     args = ["-P0", "-D0", "-d", "8", "-n", "0.0089", "-u", "0.05", "-i", "10", "-e", "1", "-w", "8,8,8", "-s", "8",
-            "-v", str(tmp_path)] + (["-o"] if optimize else [])
+            "-v", str(tmp_path)] + (["-o"] if optimize else ["--arith", "exact"])
     r = run(args, tmp_path)
     assert r.returncode == 0, r.stderr
     stats = r.stderr.strip().splitlines()[-1].split(";")
@@ -63,10 +64,11 @@ def test_make_test8(tmp_path, golden_dir, optimize):
         assert np.array_equal(vt["v"], window_v(sim.u, 8), equal_nan=True), it
 
 
-@pytest.mark.parametrize("arith", [[], ["--arith", "opencl"]])
+@pytest.mark.parametrize("arith", [["--arith", "exact"], []])
 def test_make_test32(tmp_path, golden_dir, arith):
-    """make test32; also in the reference's own rounding (--arith opencl), which
-    meets the same verify.py thresholds against the Sailfish fixtures."""
+    """make test32 in the exact arithmetic and in the default, the reference's own
+    rounding (--arith opencl): both meet verify.py's thresholds against the
+    Sailfish fixtures."""
     r = run(["-P0", "-D0", "-d", "32", "-n", "0.0089", "-u", "0.05", "-i", "500", "-e", "20", "-w", "32,32,1", "-s",
              "32", "-v", str(tmp_path)] + arith, tmp_path)
     assert r.returncode == 0, r.stderr
@@ -79,7 +81,7 @@ def test_make_testall_dumps(tmp_path, precision):
     """-f -m: every f_<it>.dump holds the lattice at the start of iteration it, map.dump the cell codes."""
     n, its, stride = 8, 6, 4
     args = ["-P0", "-D0", "-d", str(n), "-i", str(its), "-e", "1", "-w", "8,8,8", "-s", str(stride), "-v",
-            str(tmp_path), "-p", str(tmp_path), "-f", "-m"] + (["-F"] if precision == "f64" else [])
+            str(tmp_path), "-p", str(tmp_path), "-f", "-m", "--arith", "exact"] + (["-F"] if precision == "f64" else [])
     r = run(args, tmp_path)
     assert r.returncode == 0, r.stderr
     assert ("double" if precision == "f64" else "single") in r.stderr

@@ -225,13 +225,15 @@ def test_faster_than_reference_kernels(precision, n):
     assert min(ours, ours_fast, ours_ocl) > best_ref[0]
 
 
+@pytest.mark.parametrize("arith", [["--arith", "opencl"], []])
 @pytest.mark.parametrize("precision", ["f32", "f64"])
-def test_cli_opencl_arith_vti_is_the_reference_output(tmp_path, precision):
-    """`lbmcl --arith opencl` writes the VTI the reference's kernels produce on
-    this GPU: the make test32 run (32^3, 500 iterations, stride 32), last VTI."""
+def test_cli_opencl_arith_vti_is_the_reference_output(tmp_path, precision, arith):
+    """`lbmcl` (whose default is --arith opencl) writes the VTI the reference's
+    kernels produce on this GPU: the make test32 run (32^3, 500 iterations,
+    stride 32), last VTI."""
     n, iterations = 32, 500
     args = ["-P0", "-D0", "-d", str(n), "-i", str(iterations), "-e", str(iterations), "-s", "32", "-w", "32,8,1",
-            "--arith", "opencl", "-v", str(tmp_path)] + (["-F"] if precision == "f64" else [])
+            "-v", str(tmp_path)] + arith + (["-F"] if precision == "f64" else [])
     print("lbmcl", " ".join(args), flush=True)
     r = subprocess.run([native_build.CLI, *args], capture_output=True, text=True, timeout=300,
                        stdin=subprocess.DEVNULL)
