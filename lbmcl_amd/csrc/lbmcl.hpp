@@ -61,7 +61,7 @@ class LBMCL {
     bool dump_f;
     bool dump_data = false;
     int storage = LBM_STORE_NATIVE;
-    int arith = -1;  // LBM_ARITH_*; -1: from optimize
+    int arith = -1;  // LBM_ARITH_*; -1: from optimize and the storage
 
     int device_id = -1;
     std::string device_name;
@@ -214,9 +214,10 @@ class LBMCL {
     void setStorage(int s) { storage = s; }
 
     // Collision arithmetic (LBM_ARITH_*); call before setupSimulation.  By
-    // default the constructor's `optimize` selects LBM_ARITH_FUSED or
-    // LBM_ARITH_EXACT; LBM_ARITH_OPENCL reproduces the reference's own
-    // rounding on this GPU.
+    // default the constructor's `optimize` selects LBM_ARITH_FUSED, and
+    // otherwise LBM_ARITH_OPENCL, the reference's own rounding on this GPU
+    // (16-bit storage always uses the fused form); LBM_ARITH_EXACT evaluates
+    // each operation as written.
     void setArithmetic(int a) { arith = a; }
 
     // Select the device and allocate all device memory (lbmcl.hpp:392-482).
@@ -240,7 +241,9 @@ class LBMCL {
         cfg.lws[0] = int32_t(lws[0]);
         cfg.lws[1] = int32_t(lws[1]);
         cfg.lws[2] = int32_t(lws[2]);
-        cfg.optimize = arith >= 0 ? arith : (optimize ? LBM_ARITH_FUSED : LBM_ARITH_EXACT);
+        cfg.optimize = arith >= 0                                         ? arith
+                       : (optimize || storage != LBM_STORE_NATIVE) ? LBM_ARITH_FUSED
+                                                                     : LBM_ARITH_OPENCL;
         cfg.kernel = LBM_KERNEL_AUTO;
         cfg.profile = 1;
         cfg.xcd_remap = 1;
