@@ -25,6 +25,11 @@ Pinning (what checks this oracle, see tests/test_oracle.py):
   mathematics (zero force is bit-identical to no force; the force term is
   linear and conserves mass; diagnostics against numpy sums).
 
+``fast = ARITH_CONTRACTED`` evaluates kernels.cl's expressions with the
+contractions clang forms for OpenCL C (std::fma where it emits fmuladd), the
+reference's arithmetic on AMD GPUs; in FP64 that is bit-identical to the HIP
+LBM_ARITH_OPENCL form.
+
 ``ref_run`` runs the reference's own kernels.cl -- compiled offline from the
 reference's source by ``ref_build.py`` into ``oracle/_ref`` -- on the local GPU
 through the ROCm OpenCL runtime (``ref_runner.c``), for comparisons with "the
@@ -137,11 +142,16 @@ def kernel_literal(value: float, precision: str) -> float:
     return float(_libc.strtod(text.encode(), None))
 
 
+ARITH_EXACT, ARITH_FUSED, ARITH_CONTRACTED = 0, 1, 2
+
+
 def params(dim, stride, viscosity, velocity, precision="f32", threads=None, fast=False) -> Params:
+    """`fast`: False/True for the exact or fused collision, or ARITH_CONTRACTED
+    for kernels.cl's expressions contracted as clang compiles OpenCL C."""
     if threads is None:
         threads = min(16, os.cpu_count() or 1)
     return Params(dim, threads, stride, kernel_literal(viscosity, precision), kernel_literal(velocity, precision),
-                  int(bool(fast)), 0)
+                  int(fast), 0)
 
 
 def constants(p: Params, precision="f32"):
@@ -237,7 +247,8 @@ class PushSim:
 class PullSim:
     """The gather form (native, FP16- or BF16-shifted storage).
 
-    fast selects the fused collision; the shifted storage forms always use it."""
+    fast selects the fused collision (or ARITH_CONTRACTED the contracted
+    expressions); the shifted storage forms always use the fused collision."""
 
     def __init__(self, dim, stride=None, viscosity=0.0089, velocity=0.05, precision="f32",
                  storage=STORE_NATIVE, threads=None, fast=False):
@@ -249,7 +260,7 @@ class PullSim:
         self.storage = storage
         self.dt = np.float32 if precision == "f32" else np.float64
         self.st = self.dt if storage == STORE_NATIVE else np.uint16
-        self.fast = bool(fast) or storage != STORE_NATIVE
+        self.fast = ARITH_FUSED if storage != STORE_NATIVE else int(fast)
         self.p = params(dim, self.stride, viscosity, velocity, precision, threads, self.fast)
         n3 = dim ** 3
         self.A = np.empty(n3 * Q, dtype=self.st)   # read by odd iterations

@@ -84,7 +84,9 @@ struct Params {
     int64_t stride;
     double viscosity;   // value of the VISCOSITY literal
     double velocity;    // value of the VELOCITY literal
-    int32_t fast;       // nonzero: the fused collision (no counterpart in the reference)
+    int32_t fast;       // 0: kernels.cl's expressions as written; 1: the fused collision (no
+                        // counterpart in the reference); 2: kernels.cl's expressions contracted as
+                        // clang compiles OpenCL C (FP_CONTRACT ON)
     int32_t reserved;
 };
 
@@ -101,6 +103,7 @@ template <class T> struct Consts {
     T U;
     T om1;       // 1 - inv_tau
     bool fast;
+    bool contracted;
     explicit Consts(const Params& p) {
         const T one = T(1), three = T(3), half = T(0.5);
         const T nu = T(p.viscosity);
@@ -108,7 +111,8 @@ template <class T> struct Consts {
         const T tau = (three * nu) + half;
         inv_tau = one / tau;
         om1 = one - inv_tau;
-        fast = p.fast != 0;
+        fast = p.fast == 1;
+        contracted = p.fast == 2;
         const T w0 = one / T(3), w1 = one / T(18), w7 = one / T(36);
         for (int i = 0; i < Q; ++i) w[i] = (i == 0) ? w0 : (i <= 6 ? w1 : w7);
     }
@@ -120,6 +124,23 @@ template <class T>
 inline T feq(const Consts<T>& k, int i, T rho, T ux, T uy, T uz, T u2) {
     const T eu = ((ux * T(EX[i])) + (uy * T(EY[i]))) + (uz * T(EZ[i]));
     return (rho * k.w[i]) * (((T(1) + (T(3) * eu)) + ((T(4.5) * eu) * eu)) - (T(1.5) * u2));
+}
+
+// The same expressions with clang's OpenCL contractions (FP_CONTRACT ON):
+// inside one expression x + a*b and x - a*b become fma(a, b, x) and
+// fma(-a, b, x), the left operand's product taken when both are products.
+// e_i . u multiplies by 0 and +-1 only, so it is exact in either form.
+template <class T>
+inline T feq_contracted(const Consts<T>& k, int i, T rho, T ux, T uy, T uz, T u2) {
+    const T eu = ((ux * T(EX[i])) + (uy * T(EY[i]))) + (uz * T(EZ[i]));
+    const T a = std::fma(T(3), eu, T(1));
+    const T b = std::fma(T(4.5) * eu, eu, a);
+    return (rho * k.w[i]) * std::fma(-T(1.5), u2, b);
+}
+// ((ux ux) + (uy uy)) + (uz uz) (kernels.cl:390) contracted
+template <class T>
+inline T u2_contracted(T ux, T uy, T uz) {
+    return std::fma(uz, uz, std::fma(ux, ux, uy * uy));
 }
 
 // rho and u moments, kernels.cl:355, 376-378.
@@ -228,9 +249,11 @@ inline void collide(const Consts<T>& k, int32_t t, T* f, T* rho_out, T* ux_out, 
         collide_fused(k, t, f, rho, ux, uy, uz, force);
         return;
     }
-    const T u2 = ((ux * ux) + (uy * uy)) + (uz * uz);  // kernels.cl:390
+    const T u2 = k.contracted ? u2_contracted(ux, uy, uz) : ((ux * ux) + (uy * uy)) + (uz * uz);  // kernels.cl:390
+    auto eq = [&](int i) { return k.contracted ? feq_contracted(k, i, rho, ux, uy, uz, u2)
+                                               : feq(k, i, rho, ux, uy, uz, u2); };
     if (is_moving(t)) {  // kernels.cl:393-398
-        for (int i = 0; i < Q; ++i) f[i] = feq(k, i, rho, ux, uy, uz, u2);
+        for (int i = 0; i < Q; ++i) f[i] = eq(i);
     } else if (is_bounceback(t)
 #if LBMO_MUTANT == 2  // planted mistake: corners bounce back too
                || t == CORNER
@@ -256,8 +279,8 @@ inline void collide(const Consts<T>& k, int32_t t, T* f, T* rho_out, T* ux_out, 
         const T omega = k.inv_tau;
 #endif
         for (int i = 0; i < Q; ++i) {
-            const T e = feq(k, i, rho, ux, uy, uz, u2);
-            f[i] = f[i] + omega * (e - f[i]);
+            const T e = eq(i);
+            f[i] = k.contracted ? std::fma(omega, e - f[i], f[i]) : f[i] + omega * (e - f[i]);
         }
         if (force && is_fluid(t)) {
             for (int i = 0; i < Q; ++i) {
@@ -282,6 +305,11 @@ inline void init_cell(const Consts<T>& k, int32_t t, T* f, T& rho, T& ux, T& uy,
 #endif
     uy = T(0);
     uz = T(0);
+    if (k.contracted) {
+        const T u2 = u2_contracted(ux, uy, uz);
+        for (int i = 0; i < Q; ++i) f[i] = is_wall(t) ? nan_v<T>() : feq_contracted(k, i, rho, ux, uy, uz, u2);
+        return;
+    }
     const T u2 = ((ux * ux) + (uy * uy)) + (uz * uz);
     for (int i = 0; i < Q; ++i) f[i] = is_wall(t) ? nan_v<T>() : feq(k, i, rho, ux, uy, uz, u2);
 }

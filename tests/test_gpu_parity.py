@@ -303,6 +303,24 @@ def test_full_size_1024_f32_vector_matches_generic():
     assert abs(dv[0] - dg[0]) <= 1e-12 * abs(dg[0]) and abs(dv[1] - dg[1]) <= 1e-10 * abs(dg[1])
 
 
+@pytest.mark.parametrize("n,stride,kernel", [(32, 32, capi.LBM_KERNEL_AUTO), (64, 64, capi.LBM_KERNEL_AUTO),
+                                             (64, 1, capi.LBM_KERNEL_TILE), (64, 64, capi.LBM_KERNEL_GENERIC)])
+def test_opencl_arithmetic_f64_matches_contracted_oracle(n, stride, kernel):
+    """LBM_ARITH_OPENCL in FP64 (correctly rounded division, so the CPU can
+    follow it): bit-identical to the oracle's contracted form, 100 steps."""
+    g = capi.Context(n, stride=stride, precision="f64", kernel=kernel, optimize=capi.LBM_ARITH_OPENCL)
+    o = O.PullSim(n, stride=stride, precision="f64", fast=O.ARITH_CONTRACTED)
+    with g:
+        g.initialize()
+        o.initialize()
+        for it in range(1, 101):
+            g.compute(it, update_macro=(it == 100))
+            o.step(update_macro=(it == 100))
+        _assert_macro_equal(g, o, n, "f64", f"OpenCL arithmetic FP64 {n}^3 stride {stride} kernel {kernel}")
+        buf = 1 if 101 % 2 == 0 else 0
+        assert nan_equal(g.read_state(buf), o.input_buffer(101))
+
+
 def test_allocation_failure_is_an_error_and_leaves_the_device_usable():
     """1024^3 FP64 needs two 163 GB lattices, more than the GPU's HBM: lbm_create
     fails with LBM_ERR_MEMORY after releasing what it had allocated, and the next

@@ -12,7 +12,9 @@ everything that fixes the reference's behaviour independently of it:
   state, push form == pull form bit for bit;
 * planted mistakes: oracle variants with one plausible error each must fail
   the fixture check;
-* the 16-bit codecs against numpy (binary16) and torch (bfloat16).
+* the 16-bit codecs against numpy (binary16) and torch (bfloat16);
+* the contracted form (kernels.cl as clang compiles OpenCL C): push == pull,
+  and within rounding of the exact form.
 """
 import os
 import subprocess
@@ -23,7 +25,7 @@ import pytest
 import oracle as O
 from lbmcl_amd.verify import metrics
 from lbmcl_amd.vti import read_vti
-from tests.util import nan_equal, window, window_v
+from tests.util import max_rel, nan_equal, window, window_v
 
 
 def fixture_errors(golden_dir, n, iterations, every, sim_factory=None):
@@ -175,6 +177,28 @@ def test_push_equals_pull(precision, n, stride):
         a.step(True)
         b.step(True)
         assert nan_equal(a.rho, b.rho) and nan_equal(a.u, b.u), it
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_contracted_form(precision):
+    """kernels.cl's expressions with clang's OpenCL contractions (the form of the
+    reference's kernels on AMD GPUs): push == pull bit for bit as in the exact
+    form; the contractions do change bits, by rounding only."""
+    n, stride = 16, 16
+    a = O.PushSim(n, stride=stride, precision=precision, fast=O.ARITH_CONTRACTED)
+    b = O.PullSim(n, stride=stride, precision=precision, fast=O.ARITH_CONTRACTED)
+    c = O.PullSim(n, stride=stride, precision=precision)
+    for s in (a, b, c):
+        s.initialize()
+    for it in range(1, 51):
+        assert nan_equal(a.input_buffer(it), b.view(b.input_buffer(it))), it
+        a.step(True)
+        b.step(True)
+        c.step(True)
+        assert nan_equal(a.rho, b.rho) and nan_equal(a.u, b.u), it
+    assert not nan_equal(b.u, c.u)
+    tol = 1e-5 if precision == "f32" else 1e-13
+    assert max_rel(b.rho, c.rho) <= tol and max_rel(b.u, c.u) <= 10 * tol
 
 
 def test_force_zero_is_bit_identical_and_linear():
