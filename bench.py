@@ -15,9 +15,10 @@ Printed: one JSON line with MLUPS (the reference's definition,
 compute kernel (152 B per lattice update in FP32; average kernel duration from
 HIP events recorded on the kernel's stream around every launch of the timed
 region), the same figures for the 16-bit shifted-storage forms, FP64, the fused
-collision (-o) and the reference's own OpenCL rounding (LBM_ARITH_OPENCL), a
-copy kernel's sustained bandwidth for context, and the CPU oracle timed on a
-bounded sample of the same workload (cpu_baseline).
+collision (-o), the reference's own OpenCL rounding (LBM_ARITH_OPENCL), the
+128^3 configuration and a 1024^3 FP32 lattice, a copy kernel's sustained
+bandwidth for context, and the CPU oracle timed on a bounded sample of the
+same workload (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -169,6 +170,19 @@ def main():
             variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
                                                 "bytes_per_update", "steps")}
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
+        # 1024^3 FP32: 179 GB of HBM, a lattice the reference cannot index
+        if dim == 512:
+            from lbmcl_amd import capi
+            name = "f32_native_1024"
+            log(f"[bench] variant {name}")
+            try:
+                r = run_lattice(1024, 50, 5, "f32", "native", stride, args.kernel, not args.no_remap)
+                variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs",
+                                                    "frac", "bytes_per_update", "steps")}
+                log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
+            except capi.LbmError as e:  # e.g. less free HBM than the 179 GB it needs
+                variants[name] = {"skipped": str(e)}
+                log(f"[bench] {name} skipped: {e}")
 
     log("[bench] copy bandwidth")
     copy = copy_bandwidth()
