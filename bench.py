@@ -4,7 +4,9 @@
 Workload: the 512^3 cavity (nu = 0.0089, lid u = 0.05), FP32 populations in
 the CSoA layout with stride 4096 (the fastest stride measured on MI355X: 16 KB
 runs of each population; stride 64 is ~5% slower), the vector row kernel
-(8-byte lanes), exact (uncontracted) arithmetic.  A "step" is one collide-stream iteration of the
+(8-byte lanes), the default arithmetic (lbm_config.optimize = 0): kernels.cl's
+expressions as ROCm's OpenCL compiler evaluates them, i.e. the reference's
+results bit for bit.  A "step" is one collide-stream iteration of the
 whole lattice.  The timed region
 holds exactly --steps iterations enqueued back to back on the library's stream,
 bracketed by device synchronisation; the lattice is resident in HBM and already
@@ -15,10 +17,11 @@ Printed: one JSON line with MLUPS (the reference's definition,
 compute kernel (152 B per lattice update in FP32; average kernel duration from
 HIP events recorded on the kernel's stream around every launch of the timed
 region), the same figures for the 16-bit shifted-storage forms, FP64, the fused
-collision (-o), the reference's own OpenCL rounding (LBM_ARITH_OPENCL), the
-128^3 configuration and a 1024^3 FP32 lattice, a copy kernel's sustained
-bandwidth for context, and the CPU oracle timed on a bounded sample of the
-same workload (cpu_baseline).
+collision (-o), the CPU oracle's exact arithmetic (LBM_ARITH_EXACT), the
+128^3 configuration and a 1024^3 FP32 lattice, the cost of the adjoint on the
+256^3 x 2000 configuration (forward and adjoint time, their ratio, peak device
+memory), a copy kernel's sustained bandwidth for context, and the CPU oracle
+timed on a bounded sample of the same workload (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -43,11 +46,11 @@ def log(msg):
 
 def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True,
                 arith=0):
-    """`arith`: lbm_config.optimize (0 exact, 1 fused, 2 the reference's OpenCL rounding)."""
+    """`arith`: lbm_config.optimize (0 the reference's OpenCL rounding, 1 fused, 2 exact)."""
     from lbmcl_amd import capi
     stride = stride or dim ** 3
     with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=True,
-                      xcd_remap=xcd_remap, optimize=arith) as g:
+                      xcd_remap=xcd_remap, arith=arith) as g:
         g.initialize()
         it = 1
         if warmup:
@@ -111,14 +114,56 @@ def cpu_baseline(dim, steps, threads, stride):
     }
 
 
+# the sources that define the compute kernels' code
+KERNEL_SOURCES = ("lbmcl_amd/csrc/lbm_kernels.hpp", "lbmcl_amd/csrc/lbm_lattice.hpp",
+                  "lbmcl_amd/csrc/lbm_opencl_arith.hip", "lbmcl_amd/csrc/lbm_ctx.hpp")
+
+
+def kernel_sources_sha():
+    """sha256 (first 16 hex digits) of the kernel sources, in KERNEL_SOURCES order."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profiles(dim, stride, precision, storage):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, if committed."""
-    path = os.path.join(ROOT, "profiles", f"collide_stream_vec_{dim}_s{stride}_{precision}_{storage}_pmc.md")
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, as
+    committed under profiles/ together with the kernel sources' hash they were
+    measured on: (bytes, profile file, its hash).  The bytes are None when the
+    profile is missing or was measured on other kernel sources."""
+    name = f"collide_stream_vec_{dim}_s{stride}_{precision}_{storage}_pmc.md"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, name, None
     with open(path) as fh:
-        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", fh.read())
-    return float(m.group(1)) if m else None
+        text = fh.read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    h = re.search(r"kernel_sources_sha:\s*([0-9a-f]+)", text)
+    sha = h.group(1) if h else None
+    if not m or sha != kernel_sources_sha():
+        return None, name, sha
+    return float(m.group(1)), name, sha
+
+
+def adjoint_cost(dim=256, steps=2000, every=45, seed=0):
+    """The adjoint configuration: dim^3 cavity, FP32 forward and adjoint with FP64
+    gradient sums, `steps` iterations, checkpoints every `every` steps, J = mean
+    u_x on the mid-plane z = dim/2, a seeded Gaussian random body force
+    (correlation 8 cells, amplitude 1e-6).  Times from the library's events."""
+    from lbmcl_amd.adjoint import cavity_gradient, gaussian_random_field
+    g = gaussian_random_field(dim, 8.0, 1e-6, seed=seed).astype("float32")
+    t0 = time.perf_counter()
+    r = cavity_gradient(dim, steps, force=g, objective="mean_ux_plane", plane_z=dim // 2, precision="f32",
+                        checkpoint_every=every, grad_force=True)
+    wall = time.perf_counter() - t0
+    return {"dim": dim, "steps": steps, "checkpoint_every": r["segment"], "checkpoints": r["checkpoints"],
+            "objective": "mean u_x on z = %d" % (dim // 2), "forward_ms": round(r["forward_ms"], 2),
+            "adjoint_ms": round(r["adjoint_ms"], 2), "adjoint_over_forward": round(r["adjoint_ms"] / r["forward_ms"], 3),
+            "device_bytes": r["device_bytes"], "device_gib": round(r["device_bytes"] / 2 ** 30, 2),
+            "wall_s": round(wall, 2), "J": r["objective"], "dJ_dU": r["d_velocity"], "dJ_dnu": r["d_viscosity"]}
 
 
 def main():
@@ -134,6 +179,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-remap", action="store_true", help="disable the XCD-aware block order")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 row")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="iterations of the 256^3 adjoint cost measurement (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,10 +198,10 @@ def main():
 
     variants = {}
     if args.variant_steps > 0:
-        for name, prec, st, arith in (("f32_fp16", "f32", "fp16", 1), ("f32_bf16", "f32", "bf16", 1),
+        for name, prec, st, arith in (("f32_fp16", "f32", "fp16", 0), ("f32_bf16", "f32", "bf16", 0),
                                       ("f64_native", "f64", "native", 0), ("f32_native_fused", "f32", "native", 1),
-                                      ("f32_native_opencl_arith", "f32", "native", 2),
-                                      ("f64_native_opencl_arith", "f64", "native", 2)):
+                                      ("f32_native_exact_arith", "f32", "native", 2),
+                                      ("f64_native_exact_arith", "f64", "native", 2)):
             log(f"[bench] variant {name}")
             r = run_lattice(dim, args.variant_steps, max(5, args.warmup // 4), prec, st, stride, args.kernel,
                             not args.no_remap, arith)
@@ -184,6 +231,13 @@ def main():
                 variants[name] = {"skipped": str(e)}
                 log(f"[bench] {name} skipped: {e}")
 
+    adjoint = None
+    if args.adjoint_steps > 0:
+        log(f"[bench] adjoint: 256^3 x {args.adjoint_steps}, FP32, checkpoints every 45 steps")
+        adjoint = adjoint_cost(256, args.adjoint_steps, 45)
+        log(f"[bench] adjoint: forward {adjoint['forward_ms']:.0f} ms, adjoint {adjoint['adjoint_ms']:.0f} ms "
+            f"({adjoint['adjoint_over_forward']:.2f}x), {adjoint['device_gib']:.1f} GiB")
+
     log("[bench] copy bandwidth")
     copy = copy_bandwidth()
     cpu = None
@@ -192,6 +246,7 @@ def main():
         cpu = cpu_baseline(dim, args.cpu_steps, args.cpu_threads, stride)
 
     wet = (dim - 2) ** 3
+    traffic = traffic_from_profiles(dim, stride, "f32", "native")
     out = {
         "metric": f"MLUPS (D3Q19 lid-driven cavity, {dim}^3, FP32)",
         "value": round(main_run["mlups"], 1),
@@ -206,17 +261,21 @@ def main():
         "dtype": "f32",
         "data": "synthetic (deterministic lid-driven cavity initial state; no dataset)",
         "config": {"workload": f"D3Q19 BGK lid-driven cavity {dim}^3, nu=0.0089, lid u=0.05, FP32 populations, "
-                               f"CSoA stride {stride}, pull-form vector row kernel, exact arithmetic",
+                               f"CSoA stride {stride}, pull-form vector row kernel, the reference's arithmetic "
+                               "(kernels.cl as ROCm's OpenCL compiler evaluates it; lbm_config.optimize = 0)",
                    "dim": dim, "stride": stride, "wet_cells": wet, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_run["frac"], 4),
-                     "traffic": traffic_from_profiles(dim, stride, "f32", "native"),
-                     "kernel": "lbm_collide_stream_vec<float, native, 8-byte lanes, exact>",
+                     "traffic": traffic[0],
+                     "traffic_profile": {"file": "profiles/" + traffic[1], "kernel_sources_sha": traffic[2],
+                                         "current_sources_sha": kernel_sources_sha()},
+                     "kernel": "lbm_collide_stream_vec<float, native, 8-byte lanes, OpenCL arithmetic>",
                      "avg_kernel_ms": round(main_run["avg_kernel_ms"], 4),
                      "bytes_per_launch": wet * 152},
         "cpu_baseline": cpu,
         "kernel_mlups": round(main_run["kernel_mlups"], 1),
         "variants": variants,
+        "adjoint": adjoint,
         "copy_bandwidth": copy,
     }
     print(json.dumps(out), flush=True)

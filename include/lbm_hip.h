@@ -70,10 +70,17 @@ extern "C" {
  * LBM_KERNEL_AUTO selects it when stride * stored bytes <= 32. */
 #define LBM_KERNEL_TILE 5
 
-/* lbm_config.optimize: arithmetic form of the collision (see lbm_compute) */
-#define LBM_ARITH_EXACT 0   /* kernels.cl's expressions operation by operation, no contraction */
-#define LBM_ARITH_FUSED 1   /* -o: the fused BGK form (always used with 16-bit storage) */
-#define LBM_ARITH_OPENCL 2  /* kernels.cl's expressions as ROCm's OpenCL compiler evaluates them */
+/* lbm_config.optimize: arithmetic form of the collision (see lbm_compute).
+ * The zero value is the reference's own arithmetic: with native storage,
+ * kernels.cl's expressions as ROCm's OpenCL compiler evaluates them (the
+ * reference's results on this GPU, bit for bit); 16-bit storage has no
+ * counterpart in the reference and always computes in the fused form. */
+#define LBM_ARITH_REFERENCE 0 /* default: LBM_ARITH_OPENCL for native storage, LBM_ARITH_FUSED for 16-bit */
+#define LBM_ARITH_OPENCL 0    /* kernels.cl's expressions as ROCm's OpenCL compiler evaluates them */
+#define LBM_ARITH_FUSED 1     /* -o: the fused BGK form (always used with 16-bit storage) */
+#define LBM_ARITH_EXACT 2     /* the CPU oracle's form: kernels.cl's expressions operation by operation,
+                                 no contraction, correctly rounded division; a checking aid, not the
+                                 reference's rounding (native storage only) */
 
 /* event kinds, named like the reference's events (lbmcl.hpp:26-31) */
 #define LBM_EVENT_INITIALIZE 0
@@ -84,7 +91,13 @@ extern "C" {
 #define LBM_EVENT_READ_U 5
 #define LBM_EVENT_DIAGNOSTICS 6
 
+/* Both configuration structs start with their own size: set struct_size to
+ * sizeof(lbm_config) (LBM_CONFIG_INIT does) and the library rejects a struct
+ * of any other size with LBM_ERR_INVALID, so a caller built against another
+ * layout fails loudly instead of passing shifted fields.  Every other field's
+ * zero value is its default (dim and stride have none). */
 typedef struct lbm_config {
+    uint32_t struct_size; /* sizeof(lbm_config) */
     int32_t dim;         /* lattice cube dimension, a power of two (lbmcl.hpp:366-369) */
     int32_t precision;   /* LBM_F32 or LBM_F64 */
     int32_t storage;     /* LBM_STORE_*; shifted storage requires LBM_F32 */
@@ -93,7 +106,7 @@ typedef struct lbm_config {
     double viscosity;    /* kinematic viscosity nu; tau = 3 nu + 1/2 (kernels.cl:61-62) */
     double velocity;     /* lid x-velocity U (kernels.cl:57) */
     int32_t lws[3];      /* work-group size of the generic kernel (product <= 1024); 0 = default */
-    int32_t optimize;    /* LBM_ARITH_*: 1 is -o (fused collision; 16-bit storage always uses it) */
+    int32_t optimize;    /* LBM_ARITH_*: 0 the reference's arithmetic, 1 is -o (fused collision) */
     int32_t kernel;      /* LBM_KERNEL_* */
     int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
     int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernels) */
@@ -101,16 +114,22 @@ typedef struct lbm_config {
                                rounding of the reference's -D literals (for gradient checks) */
 } lbm_config;
 
+/* lbm_config cfg = LBM_CONFIG_INIT;  -- struct_size set, every other field zero */
+#define LBM_CONFIG_INIT {(uint32_t)sizeof(lbm_config)}
+
 /* objectives of lbm_adjoint (no counterpart in the reference) */
 #define LBM_OBJECTIVE_KINETIC_ENERGY 0  /* J = sum over FLUID/MOVING cells of rho |u|^2 / 2 */
 #define LBM_OBJECTIVE_MEAN_UX_PLANE 1   /* J = mean u_x over the FLUID/MOVING cells of plane z */
 
 typedef struct lbm_adjoint_config {
+    uint32_t struct_size;      /* sizeof(lbm_adjoint_config) */
     int32_t objective;         /* LBM_OBJECTIVE_* */
     int32_t plane_z;           /* plane of LBM_OBJECTIVE_MEAN_UX_PLANE, 1 .. dim-2 */
     int64_t steps;             /* T: the objective is evaluated on the state after T iterations */
     int64_t checkpoint_every;  /* K: forward states kept every K steps; 0 = round(sqrt(T)) */
 } lbm_adjoint_config;
+
+#define LBM_ADJOINT_CONFIG_INIT {(uint32_t)sizeof(lbm_adjoint_config)}
 
 typedef struct lbm_adjoint_result {
     double objective;     /* J */
@@ -154,18 +173,20 @@ int lbm_initialize(lbm_ctx* ctx);
  * diagnostics != 0 also reduces the mass sum(rho) and kinetic energy
  * sum(rho |u|^2 / 2) over FLUID/MOVING cells in FP64; read them with
  * lbm_read_diagnostics (no counterpart in the reference).
- * Arithmetic (lbm_config.optimize): LBM_ARITH_EXACT evaluates each expression
- * of kernels.cl:343-418 operation by operation without contraction.
- * LBM_ARITH_FUSED (the -o option, the reference's -cl-fast-relaxed-math, and
- * always for 16-bit storage) uses a fused BGK form, f' = (1 - 1/tau) f +
- * (rho w_i / tau) T_i with T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u,
- * evaluated with fused multiply-adds; rho and u are unchanged, results differ
- * by rounding.  LBM_ARITH_OPENCL (native storage only) evaluates the same
- * expressions as ROCm's OpenCL compiler builds kernels.cl without -o: products
- * added or subtracted within one expression are contracted into fused
- * multiply-adds (OpenCL C's default FP_CONTRACT ON, as clang forms them) and
- * FP32 divisions round to OpenCL's 2.5 ulp; lbm_initialize then uses the same
- * contracted equilibrium.  This is the reference's own arithmetic on this GPU. */
+ * Arithmetic (lbm_config.optimize): LBM_ARITH_OPENCL, the default for native
+ * storage, evaluates kernels.cl:343-418 as ROCm's OpenCL compiler builds it
+ * without -o: products added or subtracted within one expression are
+ * contracted into fused multiply-adds (OpenCL C's default FP_CONTRACT ON, as
+ * clang forms them) and FP32 divisions round to OpenCL's 2.5 ulp;
+ * lbm_initialize then uses the same contracted equilibrium.  This is the
+ * reference's own arithmetic on this GPU.  LBM_ARITH_FUSED (the -o option, the
+ * reference's -cl-fast-relaxed-math, and always for 16-bit storage) uses a
+ * fused BGK form, f' = (1 - 1/tau) f + (rho w_i / tau) T_i with
+ * T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u, evaluated with fused
+ * multiply-adds; rho and u are unchanged, results differ by rounding.
+ * LBM_ARITH_EXACT evaluates each expression operation by operation without
+ * contraction and with correctly rounded division: the CPU oracle's form, bit
+ * for bit, whose FP32 results differ from the reference's by rounding. */
 int lbm_compute(lbm_ctx* ctx, int64_t iteration, int32_t update_macro, int32_t diagnostics);
 
 /* Enqueue iterations first .. first+count-1; update_macro is set on those

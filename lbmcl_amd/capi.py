@@ -24,8 +24,13 @@ LBM_ERR_STATE = -5
 LBM_F32, LBM_F64 = 0, 1
 LBM_STORE_NATIVE, LBM_STORE_FP16_SHIFT, LBM_STORE_BF16_SHIFT = 0, 1, 2
 LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_VEC8, LBM_KERNEL_TILE = 0, 1, 2, 3, 4, 5
-# lbm_config.optimize: arithmetic form of the collision
-LBM_ARITH_EXACT, LBM_ARITH_FUSED, LBM_ARITH_OPENCL = 0, 1, 2
+# lbm_config.optimize: arithmetic form of the collision.  The zero value is the
+# reference's own arithmetic (kernels.cl as ROCm's OpenCL compiler evaluates it;
+# 16-bit storage always computes in the fused form); EXACT is the CPU oracle's.
+LBM_ARITH_REFERENCE = LBM_ARITH_OPENCL = 0
+LBM_ARITH_FUSED, LBM_ARITH_EXACT = 1, 2
+ARITH_NAMES = {"reference": LBM_ARITH_REFERENCE, "opencl": LBM_ARITH_OPENCL, "fused": LBM_ARITH_FUSED,
+               "exact": LBM_ARITH_EXACT}
 
 EVENT_NAMES = {0: "initialize", 1: "compute", 2: "read_map", 3: "read_f", 4: "read_rho", 5: "read_u",
                6: "diagnostics"}
@@ -48,6 +53,7 @@ LBM_OBJECTIVE_KINETIC_ENERGY, LBM_OBJECTIVE_MEAN_UX_PLANE = 0, 1
 
 class LbmConfig(ctypes.Structure):
     _fields_ = [
+        ("struct_size", ctypes.c_uint32),
         ("dim", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("storage", ctypes.c_int32),
@@ -66,6 +72,7 @@ class LbmConfig(ctypes.Structure):
 
 class LbmAdjointConfig(ctypes.Structure):
     _fields_ = [
+        ("struct_size", ctypes.c_uint32),
         ("objective", ctypes.c_int32),
         ("plane_z", ctypes.c_int32),
         ("steps", ctypes.c_int64),
@@ -176,8 +183,9 @@ class Context:
     """One device lattice: a thin owner of an lbm_ctx."""
 
     def __init__(self, dim, viscosity=0.0089, velocity=0.05, stride=None, precision="f32", storage="native",
-                 device=0, lws=(0, 0, 0), kernel=LBM_KERNEL_AUTO, profile=True, xcd_remap=True, optimize=False,
-                 raw_parameters=False):
+                 device=0, lws=(0, 0, 0), kernel=LBM_KERNEL_AUTO, profile=True, xcd_remap=True,
+                 arith=LBM_ARITH_REFERENCE, raw_parameters=False):
+        """`arith`: an LBM_ARITH_* value or one of "reference", "opencl", "fused", "exact"."""
         self.dim = int(dim)
         self.precision = precision
         self.storage = storage
@@ -187,6 +195,7 @@ class Context:
         else:
             self.sdtype = np.uint16
         cfg = LbmConfig()
+        cfg.struct_size = ctypes.sizeof(LbmConfig)
         cfg.dim = self.dim
         cfg.precision = LBM_F64 if precision == "f64" else LBM_F32
         cfg.storage = STORAGE_NAMES[storage]
@@ -195,8 +204,7 @@ class Context:
         cfg.viscosity = viscosity
         cfg.velocity = velocity
         cfg.lws[0], cfg.lws[1], cfg.lws[2] = lws
-        # LBM_ARITH_*; False and True select the exact and the fused (-o) form
-        cfg.optimize = int(optimize)
+        cfg.optimize = ARITH_NAMES[arith] if isinstance(arith, str) else int(arith)
         cfg.kernel = kernel
         cfg.profile = int(bool(profile))
         cfg.xcd_remap = int(bool(xcd_remap))
@@ -315,7 +323,7 @@ class Context:
     def adjoint(self, steps, objective=LBM_OBJECTIVE_KINETIC_ENERGY, plane_z=0, checkpoint_every=0,
                 grad_force=True):
         """J and dJ/dU, dJ/dnu (and dJ/dg, shape (3, dim^3), if grad_force) after `steps` iterations."""
-        cfg = LbmAdjointConfig(objective, plane_z, steps, checkpoint_every)
+        cfg = LbmAdjointConfig(ctypes.sizeof(LbmAdjointConfig), objective, plane_z, steps, checkpoint_every)
         res = LbmAdjointResult()
         g = np.zeros(3 * self.n3, dtype=np.float64) if grad_force else None
         check(lib().lbm_adjoint(self._h, ctypes.byref(cfg), ctypes.byref(res), None if g is None else _ptr(g)),

@@ -21,7 +21,7 @@
 int main(int argc, char** argv) {
     const int dim = argc > 1 ? atoi(argv[1]) : 64;
     const int iterations = argc > 2 ? atoi(argv[2]) : 100;
-    lbm_config cfg = {0};
+    lbm_config cfg = LBM_CONFIG_INIT; /* struct_size set; optimize 0 is the reference's arithmetic */
     cfg.dim = dim;
     cfg.precision = LBM_F32;
     cfg.storage = LBM_STORE_NATIVE;

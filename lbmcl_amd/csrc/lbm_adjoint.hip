@@ -280,7 +280,15 @@ int lbm_set_force(lbm_ctx* c, const void* force) {
 }
 
 int lbm_adjoint(lbm_ctx* c, const lbm_adjoint_config* cfg, lbm_adjoint_result* res, double* grad_force) {
-    if (!c || !cfg || !res) return fail(LBM_ERR_INVALID, "null argument");
+    if (!cfg || !res) return fail(LBM_ERR_INVALID, "null argument");
+    // the layout check comes first: it needs no device
+    if (cfg->struct_size != sizeof(lbm_adjoint_config))
+        return fail(LBM_ERR_INVALID, "lbm_adjoint_config.struct_size is " + std::to_string(cfg->struct_size) +
+                                         ", this library's lbm_adjoint_config has " +
+                                         std::to_string(sizeof(lbm_adjoint_config)) +
+                                         " bytes (set it to sizeof(lbm_adjoint_config), e.g. with "
+                                         "LBM_ADJOINT_CONFIG_INIT)");
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
     int rc = set_device(c);
     if (rc) return rc;
     if (c->cfg.storage != LBM_STORE_NATIVE) return fail(LBM_ERR_INVALID, "the adjoint needs native storage");

@@ -62,6 +62,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
     *out = nullptr;
     const lbm_config& k = *cfg;
+    if (k.struct_size != sizeof(lbm_config))
+        return fail(LBM_ERR_INVALID, "lbm_config.struct_size is " + std::to_string(k.struct_size) +
+                                         ", this library's lbm_config has " + std::to_string(sizeof(lbm_config)) +
+                                         " bytes (set it to sizeof(lbm_config), e.g. with LBM_CONFIG_INIT)");
     if (!is_pow2(k.dim)) return fail(LBM_ERR_INVALID, "dim must be a positive power of two");
     if (k.dim > 1024) return fail(LBM_ERR_INVALID, "dim above 1024 is not supported");
     const int64_t n3 = int64_t(k.dim) * k.dim * k.dim;
@@ -72,10 +76,11 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         return fail(LBM_ERR_INVALID, "unknown storage");
     if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
         return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
-    if (k.optimize != LBM_ARITH_EXACT && k.optimize != LBM_ARITH_FUSED && k.optimize != LBM_ARITH_OPENCL)
-        return fail(LBM_ERR_INVALID, "optimize must be LBM_ARITH_EXACT, LBM_ARITH_FUSED or LBM_ARITH_OPENCL");
-    if (k.optimize == LBM_ARITH_OPENCL && k.storage != LBM_STORE_NATIVE)
-        return fail(LBM_ERR_INVALID, "LBM_ARITH_OPENCL needs native storage (16-bit storage uses the fused form)");
+    if (k.optimize != LBM_ARITH_REFERENCE && k.optimize != LBM_ARITH_FUSED && k.optimize != LBM_ARITH_EXACT)
+        return fail(LBM_ERR_INVALID,
+                    "optimize must be LBM_ARITH_REFERENCE (LBM_ARITH_OPENCL), LBM_ARITH_FUSED or LBM_ARITH_EXACT");
+    if (k.optimize == LBM_ARITH_EXACT && k.storage != LBM_STORE_NATIVE)
+        return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the fused form)");
     if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
         k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
         return fail(LBM_ERR_INVALID, "unknown kernel selection");

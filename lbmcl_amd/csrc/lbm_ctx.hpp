@@ -342,13 +342,14 @@ inline int64_t compute_blocks(const lbm_ctx* c) {
 }
 
 // Arithmetic form of the collision: the fused form for 16-bit storage always
-// and for native storage with -o (LBM_ARITH_FUSED); LBM_ARITH_OPENCL evaluates
-// kernels.cl's expressions the way ROCm's OpenCL compiler does.
+// and for native storage with -o (LBM_ARITH_FUSED); the default
+// (LBM_ARITH_OPENCL) evaluates kernels.cl's expressions the way ROCm's OpenCL
+// compiler does; LBM_ARITH_EXACT is the CPU oracle's form.
 inline int arith_of(const lbm_ctx* c) {
     if (c->cfg.storage != LBM_STORE_NATIVE) return lbm::AR_FUSED;
     if (c->cfg.optimize == LBM_ARITH_FUSED) return lbm::AR_FUSED;
-    if (c->cfg.optimize == LBM_ARITH_OPENCL) return lbm::AR_OPENCL;
-    return lbm::AR_EXACT;
+    if (c->cfg.optimize == LBM_ARITH_EXACT) return lbm::AR_EXACT;
+    return lbm::AR_OPENCL;
 }
 
 // Compute launches in the OpenCL arithmetic form, defined in

@@ -231,6 +231,7 @@ class LBMCL {
         device_name = name;
 
         lbm_config cfg{};
+        cfg.struct_size = sizeof(lbm_config);
         cfg.dim = int32_t(dim);
         cfg.precision = std::is_same<T, float>::value ? LBM_F32 : LBM_F64;
         cfg.storage = storage;

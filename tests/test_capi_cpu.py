@@ -7,6 +7,7 @@
 * The lbmcl command line parses the reference's options and rejects bad values
   with the reference's messages (libs/lbm_options.hpp:109-161).
 """
+import ctypes
 import os
 import re
 import subprocess
@@ -58,8 +59,9 @@ def test_library_exports_every_declared_symbol():
     (dict(dim=8, viscosity=-1.0), capi.LBM_ERR_INVALID, "viscosity"),
     (dict(dim=8, velocity=float("nan")), capi.LBM_ERR_INVALID, "velocity"),
     (dict(dim=8, lws=(64, 64, 1)), capi.LBM_ERR_INVALID, "work-group"),
-    (dict(dim=8, optimize=3), capi.LBM_ERR_INVALID, "LBM_ARITH_OPENCL"),
-    (dict(dim=8, storage="bf16", optimize=capi.LBM_ARITH_OPENCL), capi.LBM_ERR_INVALID, "native storage"),
+    (dict(dim=8, arith=3), capi.LBM_ERR_INVALID, "LBM_ARITH_REFERENCE"),
+    (dict(dim=8, arith=-1), capi.LBM_ERR_INVALID, "LBM_ARITH_REFERENCE"),
+    (dict(dim=8, storage="bf16", arith=capi.LBM_ARITH_EXACT), capi.LBM_ERR_INVALID, "native storage"),
     (dict(dim=8, device=99), capi.LBM_ERR_DEVICE, "no HIP device"),
 ])
 def test_create_rejects_bad_configurations(kw, code, msg):
@@ -67,6 +69,56 @@ def test_create_rejects_bad_configurations(kw, code, msg):
         capi.Context(**kw)
     assert e.value.code == code
     assert msg in str(e.value)
+
+
+def _c_struct_sizes(tmp=[]):
+    """sizeof(lbm_config) and sizeof(lbm_adjoint_config) as a C compiler lays them out."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "s.c"), os.path.join(d, "s")
+        with open(src, "w") as fh:
+            fh.write('#include <stdio.h>\n#include "lbm_hip.h"\nint main(void) {\n'
+                     '  lbm_config c = LBM_CONFIG_INIT; lbm_adjoint_config a = LBM_ADJOINT_CONFIG_INIT;\n'
+                     '  printf("%u %u\\n", c.struct_size, a.struct_size); return 0; }\n')
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src, "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    return int(out[0]), int(out[1])
+
+
+def test_config_struct_size_is_checked():
+    """A caller built against another layout of the configuration structs is
+    rejected before anything else is read: lbm_create and lbm_adjoint compare
+    struct_size with their own sizeof."""
+    L = capi.lib()
+    assert (ctypes.sizeof(capi.LbmConfig), ctypes.sizeof(capi.LbmAdjointConfig)) == _c_struct_sizes()
+    cfg = capi.LbmConfig()
+    cfg.dim, cfg.stride, cfg.viscosity, cfg.velocity = 8, 8, 0.0089, 0.05
+    h = ctypes.c_void_p()
+    for size in (0, ctypes.sizeof(capi.LbmConfig) - 8, ctypes.sizeof(capi.LbmConfig) + 8):
+        cfg.struct_size = size
+        assert L.lbm_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.LBM_ERR_INVALID
+        err = capi._last_error()
+        assert f"struct_size is {size}" in err and f"has {ctypes.sizeof(capi.LbmConfig)} bytes" in err
+        assert not h.value
+    acfg = capi.LbmAdjointConfig(ctypes.sizeof(capi.LbmAdjointConfig) + 4, 0, 0, 10, 0)
+    res = capi.LbmAdjointResult()
+    assert L.lbm_adjoint(None, ctypes.byref(acfg), ctypes.byref(res), None) == capi.LBM_ERR_INVALID
+    assert "lbm_adjoint_config.struct_size is 36" in capi._last_error()
+    acfg.struct_size = ctypes.sizeof(capi.LbmAdjointConfig)
+    assert L.lbm_adjoint(None, ctypes.byref(acfg), ctypes.byref(res), None) == capi.LBM_ERR_INVALID
+    assert "null context" in capi._last_error()
+
+
+def test_zero_config_defaults_to_reference_arithmetic():
+    """lbm_config's zero `optimize` is the reference's arithmetic, and the C
+    header agrees with the Python constants."""
+    hdr = open(os.path.join(ROOT, "include", "lbm_hip.h")).read()
+    for name in ("LBM_ARITH_REFERENCE", "LBM_ARITH_OPENCL", "LBM_ARITH_FUSED", "LBM_ARITH_EXACT"):
+        m = re.search(rf"#define {name} (\d+)", hdr)
+        assert m and int(m.group(1)) == getattr(capi, name), name
+    assert capi.LBM_ARITH_REFERENCE == capi.LBM_ARITH_OPENCL == 0
+    assert capi.LbmConfig().optimize == 0
 
 
 def test_null_arguments_are_errors():

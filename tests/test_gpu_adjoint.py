@@ -7,7 +7,12 @@
   finite differences (h = 1e-6 relative) on seeded random directions of
   (U, nu, g(x)) -- the 64^3 x 500 configuration with a Gaussian random body
   force (correlation 8, amplitude 1e-6) and J = total kinetic energy, and the
-  plane objective; the FP32 adjoint against the same FP64 differences.
+  plane objective; the FP32 adjoint against the same FP64 differences, held
+  to the same 1e-4;
+* the large configuration: 256^3 x 2000 in FP32 with checkpoints every 45
+  steps and J = mean u_x on the mid-plane z = 128 -- the adjoint costs at most
+  3x the forward pass, fits in the GPU's memory, gives the same gradients with
+  checkpoints every 50 steps, and agrees with FP64 central differences.
 """
 import numpy as np
 import pytest
@@ -28,8 +33,9 @@ def test_force_matches_oracle(precision, kernel):
     stride = 2 if kernel == capi.LBM_KERNEL_TILE else n
     dt = np.float32 if precision == "f32" else np.float64
     g = gaussian_random_field(n, 4.0, 1e-4, seed=5).astype(dt)
-    with capi.Context(n, stride=stride, precision=precision, kernel=kernel) as a, \
-            capi.Context(n, stride=stride, precision=precision, kernel=kernel) as b:
+    ex = capi.LBM_ARITH_EXACT  # the oracle's form
+    with capi.Context(n, stride=stride, precision=precision, kernel=kernel, arith=ex) as a, \
+            capi.Context(n, stride=stride, precision=precision, kernel=kernel, arith=ex) as b:
         o = O.PullSim(n, stride=stride, precision=precision)
         o.force = g.reshape(-1)
         a.set_force(np.zeros(3 * n ** 3, dtype=dt))
@@ -37,7 +43,7 @@ def test_force_matches_oracle(precision, kernel):
         a.initialize()
         b.initialize()
         o.initialize()
-        ref = capi.Context(n, stride=stride, precision=precision, kernel=kernel)
+        ref = capi.Context(n, stride=stride, precision=precision, kernel=kernel, arith=ex)
         ref.initialize()
         for it in range(1, 7):
             a.compute(it, update_macro=True)
@@ -111,7 +117,7 @@ def test_gradient_matches_fp64_finite_differences(objective):
     r32 = cavity_gradient(n, steps, U, nu, g, precision="f32", **kw)
     rng = np.random.default_rng(123)
     worst64 = worst32 = 0.0
-    ndir = 16 if objective == "kinetic_energy" else 4
+    ndir = 16
     for d in range(ndir):
         dU, dnu, dg = _direction(n, rng)
         fd = _fd(n, steps, U, nu, g, dU, dnu, dg, 1e-6, **kw)
@@ -123,7 +129,7 @@ def test_gradient_matches_fp64_finite_differences(objective):
     print(f"{objective}: worst relative error FP64 {worst64:.3e}, FP32 {worst32:.3e}; "
           f"adjoint/forward time {r64['adjoint_ms'] / r64['forward_ms']:.2f} (FP64)")
     assert worst64 < 1e-4
-    assert worst32 < 1e-2
+    assert worst32 < 1e-4
     # the body-force part alone (directions in g only; larger step, J is nearly linear in g)
     for d in range(4):
         dg = gaussian_random_field(n, 8.0, 1.0, 1000 + d)
@@ -131,6 +137,39 @@ def test_gradient_matches_fp64_finite_differences(objective):
         a64 = _adjoint_directional(r64, U, nu, 0.0, 0.0, dg)
         print(f"{objective} g-only dir {d}: fd {fd:+.10e} adj64 {a64:+.10e} ({abs(a64 - fd) / abs(fd):.2e})")
         assert abs(a64 - fd) <= 1e-4 * abs(fd)
+
+
+def test_large_checkpointed_adjoint_cost_and_gradient():
+    """256^3 x 2000, FP32 forward and adjoint (FP64 gradient sums), K = 45,
+    J = mean u_x on z = 128, seeded Gaussian random body force."""
+    torch = pytest.importorskip("torch")
+    n, steps, U, nu = 256, 2000, 0.05, 0.0089
+    g = gaussian_random_field(n, 8.0, 1e-6, seed=0)
+    kw = dict(objective="mean_ux_plane", plane_z=128)
+    print("256^3 x 2000 FP32 adjoint, checkpoints every 45 steps", flush=True)
+    r45 = cavity_gradient(n, steps, U, nu, g.astype(np.float32), precision="f32", checkpoint_every=45, **kw)
+    ratio = r45["adjoint_ms"] / r45["forward_ms"]
+    hbm = torch.cuda.get_device_properties(0).total_memory
+    print(f"forward {r45['forward_ms']:.0f} ms, adjoint {r45['adjoint_ms']:.0f} ms ({ratio:.2f}x), "
+          f"{r45['checkpoints']} checkpoints, {r45['device_bytes'] / 2 ** 30:.1f} GiB of {hbm / 2 ** 30:.0f} GiB",
+          flush=True)
+    assert r45["checkpoints"] == 45 and r45["segment"] == 45
+    assert ratio <= 3.0
+    assert r45["device_bytes"] < hbm
+    print("the same with checkpoints every 50 steps", flush=True)
+    r50 = cavity_gradient(n, steps, U, nu, g.astype(np.float32), precision="f32", checkpoint_every=50, **kw)
+    for k in ("objective", "d_velocity", "d_viscosity"):
+        assert r50[k] == r45[k], k
+    assert np.array_equal(r50["d_force"], r45["d_force"])
+    rng = np.random.default_rng(2024)
+    for d in range(2):
+        dU, dnu, dg = _direction(n, rng)
+        print(f"FP64 central differences, direction {d}", flush=True)
+        fd = _fd(n, steps, U, nu, g, dU, dnu, dg, 1e-6, precision="f64", **kw)
+        a32 = _adjoint_directional(r45, U, nu, dU, dnu, dg)
+        err = abs(a32 - fd) / abs(fd)
+        print(f"direction {d}: fd {fd:+.10e} adjoint (FP32) {a32:+.10e} ({err:.2e})", flush=True)
+        assert err < 1e-4
 
 
 def test_autograd_function():

@@ -16,7 +16,7 @@ import oracle as O
 from lbmcl_amd import capi
 from lbmcl_amd.verify import metrics
 from lbmcl_amd.vti import read_vti
-from tests.util import max_rel, mismatch_count, nan_equal, window, window_v
+from tests.util import hip_arith, max_rel, mismatch_count, nan_equal, window, window_v
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +27,7 @@ STORE = {"native": O.STORE_NATIVE, "fp16": O.STORE_FP16, "bf16": O.STORE_BF16}
 def _pair(n, stride, precision, storage="native", kernel=capi.LBM_KERNEL_AUTO, lws=(0, 0, 0),
           viscosity=0.0089, velocity=0.05, fast=False):
     g = capi.Context(n, viscosity=viscosity, velocity=velocity, stride=stride, precision=precision,
-                     storage=storage, kernel=kernel, lws=lws, optimize=fast)
+                     storage=storage, kernel=kernel, lws=lws, arith=hip_arith(fast))
     o = O.PullSim(n, stride=stride, viscosity=viscosity, velocity=velocity, precision=precision,
                   storage=STORE[storage], fast=fast)
     g.initialize()
@@ -265,7 +265,7 @@ def test_state_roundtrip():
 def test_full_size_512_f32():
     """512^3 FP32, the bandwidth-scale configuration: 2 steps bit-identical to the oracle."""
     n = 512
-    g = capi.Context(n, stride=n ** 3, precision="f32")
+    g = capi.Context(n, stride=n ** 3, precision="f32", arith=capi.LBM_ARITH_EXACT)
     o = O.PullSim(n, stride=n ** 3, precision="f32")
     with g:
         g.initialize()
@@ -308,7 +308,7 @@ def test_full_size_1024_f32_vector_matches_generic():
 def test_opencl_arithmetic_f64_matches_contracted_oracle(n, stride, kernel):
     """LBM_ARITH_OPENCL in FP64 (correctly rounded division, so the CPU can
     follow it): bit-identical to the oracle's contracted form, 100 steps."""
-    g = capi.Context(n, stride=stride, precision="f64", kernel=kernel, optimize=capi.LBM_ARITH_OPENCL)
+    g = capi.Context(n, stride=stride, precision="f64", kernel=kernel)  # the default: LBM_ARITH_OPENCL
     o = O.PullSim(n, stride=stride, precision="f64", fast=O.ARITH_CONTRACTED)
     with g:
         g.initialize()

@@ -6,6 +6,13 @@ import numpy as np
 Q = 19
 
 
+def hip_arith(fast):
+    """The lbm_config.optimize value of the oracle's arithmetic form `fast`
+    (False: exact, True: fused, 2: contracted as ROCm's OpenCL compiler does)."""
+    from lbmcl_amd import capi
+    return {0: capi.LBM_ARITH_EXACT, 1: capi.LBM_ARITH_FUSED, 2: capi.LBM_ARITH_OPENCL}[int(fast)]
+
+
 def window(a, n):
     """The VTI point window x, y, z in [1, N-2] of a flat N^3 field (lbmcl.hpp:284-286)."""
     return np.asarray(a).reshape(n, n, n)[1:n - 1, 1:n - 1, 1:n - 1].reshape(-1)
