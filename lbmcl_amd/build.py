@@ -3,7 +3,6 @@
 Outputs (git-ignored, they travel with the tree to the GPU host):
   lbmcl_amd/liblbm_hip.so      HIP kernels + C ABI (include/lbm_hip.h), gfx950
   lbmcl_amd/bin/lbmcl          the lbmcl command line (C++ host, links liblbm_hip.so)
-  lbmcl_amd/bin/lbm_io_selftest  writes the dump formats from synthetic arrays (CPU only)
   lbmcl_amd/bin/capi_example   the C ABI used from plain C (csrc/examples/capi_example.c)
   lbmcl_amd/libmembw.so        plain HBM copy kernel used to calibrate bandwidth figures
 
@@ -36,7 +35,6 @@ HIP_FLAGS = [
 LIB = os.path.join(PKG, "liblbm_hip.so")
 MEMBW = os.path.join(PKG, "libmembw.so")
 CLI = os.path.join(BIN, "lbmcl")
-IO_SELFTEST = os.path.join(BIN, "lbm_io_selftest")
 CAPI_EXAMPLE = os.path.join(BIN, "capi_example")
 
 
@@ -69,7 +67,6 @@ def build_cli():
     common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", INCLUDE, "-I", CSRC]
     _run([cxx, *common, os.path.join(CSRC, "main.cpp"), "-o", CLI,
           "-L", PKG, "-llbm_hip", "-Wl,-rpath,$ORIGIN/.."])
-    _run([cxx, *common, os.path.join(CSRC, "lbm_io_selftest.cpp"), "-o", IO_SELFTEST])
     cc = shutil.which("gcc") or "gcc"
     _run([cc, "-std=c99", "-O2", "-Wall", "-Wextra", "-pedantic", "-I", INCLUDE,
           os.path.join(CSRC, "examples", "capi_example.c"), "-o", CAPI_EXAMPLE,

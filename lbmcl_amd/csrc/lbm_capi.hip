@@ -316,28 +316,15 @@ int lbm_snapshot(lbm_ctx* c, int32_t what, int32_t buffer, int64_t* ticket) {
             return fail(LBM_ERR_INVALID, "unknown snapshot kind");
     }
     if (!c->copy_stream) LBM_HIP_CHECK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
-    SnapSlot* s = nullptr;
+    int idx = -1;
     int64_t t = 0;
-    {
-        std::unique_lock<std::mutex> lk(c->snap_mu);
-        for (;;) {
-            for (SnapSlot& x : c->snaps)
-                if (x.state == 0) {
-                    s = &x;
-                    break;
-                }
-            if (s) break;
-            c->snap_cv.wait(lk);
-        }
-        t = c->next_ticket++;
-        s->ticket = t;
-        s->state = 1;
-    }
+    // blocks while every slot is held; fails at once after a failed wait, whose
+    // slot may never be released
+    if (c->snap_table.take(&idx, &t) != SlotTable<LBM_SNAPSHOT_SLOTS>::OK)
+        return fail(LBM_ERR_STATE, "a snapshot copy failed earlier on this context; no new snapshots");
+    SnapSlot* s = &c->snaps[idx];
     auto give_back = [&](int rc) {
-        std::lock_guard<std::mutex> lk(c->snap_mu);
-        s->state = 0;
-        s->ticket = -1;
-        c->snap_cv.notify_all();
+        c->snap_table.give_back(idx);
         return rc;
     };
     if (s->cap < bytes) {
@@ -389,18 +376,14 @@ int lbm_snapshot(lbm_ctx* c, int32_t what, int32_t buffer, int64_t* ticket) {
 int lbm_snapshot_wait(lbm_ctx* c, int64_t ticket, const void** data, int64_t* bytes) {
     LBM_CHECK_CTX(c);
     if (!data || !bytes) return fail(LBM_ERR_INVALID, "null output");
-    SnapSlot* s = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(c->snap_mu);
-        for (SnapSlot& x : c->snaps)
-            if (x.ticket == ticket && x.state != 0) s = &x;
-    }
-    if (!s) return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
-    LBM_HIP_CHECK(hipEventSynchronize(s->copied), "hipEventSynchronize");
-    {
-        std::lock_guard<std::mutex> lk(c->snap_mu);
-        s->state = 2;
-    }
+    const int idx = c->snap_table.find(ticket);
+    if (idx < 0) return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
+    SnapSlot* s = &c->snaps[idx];
+    const hipError_t e = hipEventSynchronize(s->copied);
+    // a failed copy leaves the slot FAILED: lbm_snapshot_release still frees it,
+    // and later lbm_snapshot calls report the failure instead of waiting
+    c->snap_table.waited(idx, e == hipSuccess);
+    if (e != hipSuccess) return fail(LBM_ERR_RUNTIME, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
     *data = s->host;
     *bytes = int64_t(s->bytes);
     return LBM_OK;
@@ -408,16 +391,12 @@ int lbm_snapshot_wait(lbm_ctx* c, int64_t ticket, const void** data, int64_t* by
 
 int lbm_snapshot_release(lbm_ctx* c, int64_t ticket) {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
-    std::lock_guard<std::mutex> lk(c->snap_mu);
-    for (SnapSlot& x : c->snaps)
-        if (x.ticket == ticket && x.state != 0) {
-            if (x.state == 1) return fail(LBM_ERR_STATE, "snapshot released before lbm_snapshot_wait");
-            x.state = 0;
-            x.ticket = -1;
-            c->snap_cv.notify_all();
-            return LBM_OK;
-        }
-    return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
+    switch (c->snap_table.release(ticket)) {
+        case SlotTable<LBM_SNAPSHOT_SLOTS>::OK: return LBM_OK;
+        case SlotTable<LBM_SNAPSHOT_SLOTS>::NOT_WAITED:
+            return fail(LBM_ERR_STATE, "snapshot released before lbm_snapshot_wait");
+        default: return fail(LBM_ERR_INVALID, "unknown or released snapshot ticket");
+    }
 }
 
 int lbm_read_state(lbm_ctx* c, int32_t buffer, void* out) {

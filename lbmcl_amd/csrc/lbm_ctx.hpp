@@ -21,6 +21,7 @@
 
 #include "../../include/lbm_hip.h"
 #include "lbm_kernels.hpp"
+#include "lbm_slots.hpp"
 
 namespace lbm_host {
 
@@ -89,7 +90,8 @@ struct EventLog {
 };
 
 // One staging slot of the asynchronous snapshots: device buffer, pinned host
-// buffer, and the event that marks the end of its device-to-host copy.
+// buffer, and the event that marks the end of its device-to-host copy.  Its
+// state and ticket live in the context's SlotTable.
 struct SnapSlot {
     void* dev = nullptr;
     void* host = nullptr;
@@ -97,8 +99,6 @@ struct SnapSlot {
     size_t bytes = 0;      // bytes of the current snapshot
     hipEvent_t copied = nullptr;
     hipEvent_t packed = nullptr;
-    int64_t ticket = -1;
-    int state = 0;         // 0 free, 1 enqueued, 2 waited (held by the caller)
 };
 
 }  // namespace lbm_host
@@ -125,12 +125,10 @@ struct lbm_ctx {
     double nu = 0, inv_tau = 0, U = 0, scale = 1;
     void* force = nullptr;     // body force, 3 N^3 of the arithmetic type (null: none)
     lbm_host::EventLog events;
-    // asynchronous snapshots (slot table guarded by snap_mu)
+    // asynchronous snapshots: staging slots and their state machine
     hipStream_t copy_stream = nullptr;
     lbm_host::SnapSlot snaps[LBM_SNAPSHOT_SLOTS];
-    int64_t next_ticket = 1;
-    std::mutex snap_mu;
-    std::condition_variable snap_cv;
+    lbm_host::SlotTable<LBM_SNAPSHOT_SLOTS> snap_table;
 };
 
 namespace lbm_host {

@@ -8,6 +8,11 @@
 // storage the default is opencl: the reference's kernels.cl evaluated as
 // ROCm's OpenCL compiler does, so the outputs are those of the reference on
 // the same GPU, bit for bit; exact evaluates each operation as written.
+//
+// Portions restate the user interface of LBMCL (blackwut/LBMCL, GPL-3.0):
+// the option table, help text, defaults and error messages of
+// libs/lbm_options.hpp:13-196.  This file is distributed under the GNU General Public License
+// version 3; see LICENSE at the repository root.
 #pragma once
 
 #include <getopt.h>

@@ -291,6 +291,7 @@ class Writer {
             std::string what;
             if (rc) {
                 what = std::string("lbm_snapshot_wait - ") + lbm_last_error();
+                (void)lbm_snapshot_release(ctx_, job.ticket);  // the failed slot goes back too
             } else {
                 if (job.kind == VTI) {
                     if (fp64_) write_vti<double>(job.path, static_cast<const double*>(data), dim_, threads_);

@@ -6,12 +6,18 @@
 // C ABI of include/lbm_hip.h.  A nonzero status from the library prints
 // `file:line what(code) - message` to stderr and exits with the status, as
 // CLUCheckErrorExit does (libs/CLUtil.hpp:101-117).
+//
+// Portions restate the user interface of LBMCL (blackwut/LBMCL, GPL-3.0):
+// the class interface, the constructor's checks and messages, the
+// printed configuration and the statistics line of lbmcl.hpp:33-679.  This file is distributed under the GNU General Public License
+// version 3; see LICENSE at the repository root.
 #pragma once
 
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <iomanip>
 #include <iostream>
 #include <limits>
 #include <memory>
@@ -88,23 +94,12 @@ class LBMCL {
     size_t device_memory_size_k() const { return device_memory_size_b() / (1 << 10); }
     size_t device_memory_size_m() const { return device_memory_size_b() / (1 << 20); }
 
-    static bool is_power_of_two(size_t x) { return x && !(x & (x - 1)); }
-    static size_t log2i(size_t x) {
-        size_t n;
-        for (n = 0; x > 1; x >>= 1, n++) {
-        }
-        return n;
-    }
-    static size_t previous_power_of_two(size_t x) {
-        x |= x >> 1;
-        x |= x >> 2;
-        x |= x >> 4;
-        x |= x >> 8;
-        x |= x >> 16;
-        x |= x >> 32;
-        x = x + 1;
-        return (x >> 1);
-    }
+    // powers of two through the bit width of the value
+    static int bit_width(size_t x) { return x ? 64 - __builtin_clzll(uint64_t(x)) : 0; }
+    static bool is_power_of_two(size_t x) { return x != 0 && (x & (x - 1)) == 0; }
+    static size_t log2i(size_t x) { return x > 1 ? size_t(bit_width(x) - 1) : 0; }
+    // the largest power of two <= x (0 for 0)
+    static size_t previous_power_of_two(size_t x) { return x ? size_t(1) << (bit_width(x) - 1) : 0; }
 
     // What the reference passes to its JIT build (lbmcl.hpp:131-156); here the
     // same values configure the precompiled kernels.

@@ -1,6 +1,10 @@
 // main.cpp -- the lbmcl command line (main.cpp:1-51 of the reference): parse
 // the options, run the cavity in float or double, print the timings and the
 // statistics line.
+//
+// Portions restate the user interface of LBMCL (blackwut/LBMCL, GPL-3.0):
+// the run sequence and the timing lines of main.cpp:7-51.  This file is distributed under the GNU General Public License
+// version 3; see LICENSE at the repository root.
 #include <iostream>
 
 #include "lbm_options.hpp"

@@ -342,9 +342,12 @@ void push_initialize(const Params& p, T* f_stream, T* f_collide, T* rho, T* u, i
             }
 }
 
+// writes (may be null): one counter per slot of f_stream, incremented by every
+// store of this step; the push scheme writes each slot at most once
+// (kernels.cl:423: slot (d, i) is written only by its source d - e_i).
 template <class T>
 void push_compute(const Params& p, T* f_stream, const T* f_collide, T* rho, T* u, const int32_t* map,
-                  int update_macro, const T* force) {
+                  int update_macro, const T* force, int32_t* writes = nullptr) {
     const int64_t N = p.dim, N3 = N * N * N, S = p.stride;
     const Consts<T> k(p);
 #pragma omp parallel for num_threads(p.threads) schedule(static)
@@ -375,6 +378,10 @@ void push_compute(const Params& p, T* f_stream, const T* f_collide, T* rho, T* u
                 for (int i = 0; i < Q; ++i) {  // kernels.cl:423
                     const int64_t d = (x + EX[i]) + (y + EY[i]) * N + (z + EZ[i]) * N * N;
                     f_stream[slot(d, i, S)] = f[i];
+                    if (writes) {
+#pragma omp atomic
+                        ++writes[slot(d, i, S)];
+                    }
                 }
             }
 }
@@ -516,9 +523,10 @@ void pull_initialize(const Params& p, int storage, void* Pa, void* Pb, T* rho, T
                 }
 }
 
+// writes (may be null): per-slot store counters of Pout, as in push_compute.
 template <class T>
 void pull_compute(const Params& p, int storage, const void* Pin, void* Pout, T* rho, T* u, int update_macro,
-                  const T* force, double* diag) {
+                  const T* force, double* diag, int32_t* writes = nullptr) {
     const int64_t N = p.dim, N3 = N * N * N, S = p.stride;
     const Consts<T> k(p);
     const Codec<T> cd(storage, k, p.velocity);
@@ -555,7 +563,13 @@ void pull_compute(const Params& p, int storage, const void* Pin, void* Pout, T* 
                     mass += dr;
                     ke += 0.5 * dr * (((dx * dx) + (dy * dy)) + (dz * dz));
                 }
-                for (int i = 0; i < Q; ++i) cd.store(Pout, slot(id, i, S), i, f[i]);
+                for (int i = 0; i < Q; ++i) {
+                    cd.store(Pout, slot(id, i, S), i, f[i]);
+                    if (writes) {
+#pragma omp atomic
+                        ++writes[slot(id, i, S)];
+                    }
+                }
             }
     if (diag) {
         diag[0] = mass;
@@ -637,6 +651,17 @@ void lbmo_compute_f32(const lbmo_params* p, float* fs, const float* fc, float* r
 void lbmo_compute_f64(const lbmo_params* p, double* fs, const double* fc, double* rho, double* u,
                       const int32_t* map, int32_t update_macro, const double* force) {
     push_compute<double>(*p, fs, fc, rho, u, map, update_macro, force);
+}
+
+// One push / pull step in FP64 that also counts the stores into each slot of
+// the written lattice (writes: 19 N^3 counters, added to).
+void lbmo_compute_counted_f64(const lbmo_params* p, double* fs, const double* fc, double* rho, double* u,
+                              const int32_t* map, int32_t* writes) {
+    push_compute<double>(*p, fs, fc, rho, u, map, 0, nullptr, writes);
+}
+void lbmo_pull_compute_counted_f64(const lbmo_params* p, const void* pin, void* pout, double* rho, double* u,
+                                   int32_t* writes) {
+    pull_compute<double>(*p, STORE_NATIVE, pin, pout, rho, u, 0, nullptr, nullptr, writes);
 }
 
 void lbmo_pull_initialize_f32(const lbmo_params* p, int32_t storage, void* pa, void* pb, float* rho, float* u,

@@ -1,7 +1,7 @@
 // lbm_io_selftest.cpp -- writes the three dump formats from deterministic
 // synthetic arrays, plus the arrays themselves (rho.bin, u.bin, f.bin,
 // map.bin), so the formats can be checked without a GPU.  Each file is
-// written twice: by the iostream writers (lbm_io.hpp) and, under the prefix
+// written twice: by the iostream writers (lbm_io_reference.hpp) and, under the prefix
 // "fast_", by the threaded writers of the output path (lbm_output.hpp) from
 // the snapshot layouts the device produces (packed VTI window, cell-major f).
 // The arrays include NaN of both signs, infinities, signed zero, subnormals,
@@ -17,7 +17,7 @@
 #include <string>
 #include <vector>
 
-#include "lbm_io.hpp"
+#include "lbm_io_reference.hpp"
 #include "lbm_output.hpp"
 
 // special values planted at fixed positions

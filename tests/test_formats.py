@@ -1,10 +1,12 @@
 """The dump formats of the lbmcl host (CPU only).
 
-lbmcl_amd/csrc/lbm_io.hpp writes map.dump, f_<it>.dump and lbmcl.<it>.vti.
-Here lbm_io_selftest writes them from synthetic arrays (NaNs included) and the
-test formats the same arrays with an independent Python statement of the
-reference writers (lbmcl.hpp:159-334), byte for byte.  The numpy VTI reader
-must read the result back exactly.
+lbmcl_amd/csrc/lbm_output.hpp writes map.dump, f_<it>.dump and lbmcl.<it>.vti
+for the lbmcl host.  tests/native/lbm_io_selftest writes them from synthetic
+arrays (NaNs included), with those writers and with an iostream statement of
+the reference writers (tests/native/lbm_io_reference.hpp), and the test
+formats the same arrays with an independent Python statement of the reference
+writers (lbmcl.hpp:159-334): all three agree byte for byte.  The numpy VTI
+reader must read the result back exactly.
 """
 import math
 import os
@@ -13,10 +15,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from lbmcl_amd import build as native_build
 from lbmcl_amd.vti import read_vti
-
-SELFTEST = native_build.IO_SELFTEST
+from tests.native import build_native
 
 
 def _c_float(fmt, x):
@@ -102,9 +102,7 @@ def _vti(rho, u, n, dtype_name):  # lbmcl.hpp:261-334
 
 @pytest.fixture(scope="module")
 def selftest_bin():
-    if not os.path.exists(SELFTEST):
-        native_build.build_cli()
-    return SELFTEST
+    return build_native.build("lbm_io_selftest")
 
 
 @pytest.mark.parametrize("n,stride,prec,it,its,threads", [(8, 4, "f32", 7, 120, 3), (8, 512, "f64", 3, 3, 1),
