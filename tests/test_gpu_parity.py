@@ -26,8 +26,10 @@ STORE = {"native": O.STORE_NATIVE, "fp16": O.STORE_FP16, "bf16": O.STORE_BF16}
 
 def _pair(n, stride, precision, storage="native", kernel=capi.LBM_KERNEL_AUTO, lws=(0, 0, 0),
           viscosity=0.0089, velocity=0.05, fast=False):
+    # 16-bit storage has one arithmetic form (the zero default); the oracle uses it for any `fast`
+    arith = hip_arith(fast) if storage == "native" else capi.LBM_ARITH_REFERENCE
     g = capi.Context(n, viscosity=viscosity, velocity=velocity, stride=stride, precision=precision,
-                     storage=storage, kernel=kernel, lws=lws, arith=hip_arith(fast))
+                     storage=storage, kernel=kernel, lws=lws, arith=arith)
     o = O.PullSim(n, stride=stride, viscosity=viscosity, velocity=velocity, precision=precision,
                   storage=STORE[storage], fast=fast)
     g.initialize()
