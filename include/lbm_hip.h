@@ -74,10 +74,11 @@ extern "C" {
  * The zero value is the reference's own arithmetic: with native storage,
  * kernels.cl's expressions as ROCm's OpenCL compiler evaluates them (the
  * reference's results on this GPU, bit for bit); 16-bit storage has no
- * counterpart in the reference and always computes in the fused form. */
+ * counterpart in the reference and always computes in its own deviation form
+ * (see lbm_compute), whatever optimize says (LBM_ARITH_EXACT is rejected). */
 #define LBM_ARITH_REFERENCE 0 /* default: LBM_ARITH_OPENCL for native storage, LBM_ARITH_FUSED for 16-bit */
 #define LBM_ARITH_OPENCL 0    /* kernels.cl's expressions as ROCm's OpenCL compiler evaluates them */
-#define LBM_ARITH_FUSED 1     /* -o: the fused BGK form (always used with 16-bit storage) */
+#define LBM_ARITH_FUSED 1     /* -o: the fused BGK form (native storage) */
 #define LBM_ARITH_EXACT 2     /* the CPU oracle's form: kernels.cl's expressions operation by operation,
                                  no contraction, correctly rounded division; a checking aid, not the
                                  reference's rounding (native storage only) */
@@ -180,13 +181,18 @@ int lbm_initialize(lbm_ctx* ctx);
  * clang forms them) and FP32 divisions round to OpenCL's 2.5 ulp;
  * lbm_initialize then uses the same contracted equilibrium.  This is the
  * reference's own arithmetic on this GPU.  LBM_ARITH_FUSED (the -o option, the
- * reference's -cl-fast-relaxed-math, and always for 16-bit storage) uses a
+ * reference's -cl-fast-relaxed-math) uses a
  * fused BGK form, f' = (1 - 1/tau) f + (rho w_i / tau) T_i with
  * T_i = (1 - 1.5 u^2) + e (3 + 4.5 e), e = e_i . u, evaluated with fused
  * multiply-adds; rho and u are unchanged, results differ by rounding.
  * LBM_ARITH_EXACT evaluates each expression operation by operation without
  * contraction and with correctly rounded division: the CPU oracle's form, bit
- * for bit, whose FP32 results differ from the reference's by rounding. */
+ * for bit, whose FP32 results differ from the reference's by rounding.
+ * 16-bit storage computes in FP32 on the deviations d_i = f_i - w_i that its
+ * lattice stores: rho = 1 + sum d_i, j = sum e_i d_i, and the BGK update
+ * d_i' = (w_i / tau) (rho tau_i + (rho - 1)) + (1 - 1/tau) d_i with
+ * tau_i = T_i - 1, so no population is ever rounded next to w_i in FP32
+ * (which biases the mass by ~3e-7 of itself per step with 16-bit inputs). */
 int lbm_compute(lbm_ctx* ctx, int64_t iteration, int32_t update_macro, int32_t diagnostics);
 
 /* Enqueue iterations first .. first+count-1; update_macro is set on those

@@ -62,53 +62,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
     *out = nullptr;
     const lbm_config& k = *cfg;
-    if (k.struct_size != sizeof(lbm_config))
-        return fail(LBM_ERR_INVALID, "lbm_config.struct_size is " + std::to_string(k.struct_size) +
-                                         ", this library's lbm_config has " + std::to_string(sizeof(lbm_config)) +
-                                         " bytes (set it to sizeof(lbm_config), e.g. with LBM_CONFIG_INIT)");
-    if (!is_pow2(k.dim)) return fail(LBM_ERR_INVALID, "dim must be a positive power of two");
-    if (k.dim > 1024) return fail(LBM_ERR_INVALID, "dim above 1024 is not supported");
-    const int64_t n3 = int64_t(k.dim) * k.dim * k.dim;
-    if (!is_pow2(k.stride) || k.stride > n3)
-        return fail(LBM_ERR_INVALID, "stride must be a power of two in [1, dim^3]");
-    if (k.precision != LBM_F32 && k.precision != LBM_F64) return fail(LBM_ERR_INVALID, "unknown precision");
-    if (k.storage != LBM_STORE_NATIVE && k.storage != LBM_STORE_FP16_SHIFT && k.storage != LBM_STORE_BF16_SHIFT)
-        return fail(LBM_ERR_INVALID, "unknown storage");
-    if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
-        return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
-    if (k.optimize != LBM_ARITH_REFERENCE && k.optimize != LBM_ARITH_FUSED && k.optimize != LBM_ARITH_EXACT)
-        return fail(LBM_ERR_INVALID,
-                    "optimize must be LBM_ARITH_REFERENCE (LBM_ARITH_OPENCL), LBM_ARITH_FUSED or LBM_ARITH_EXACT");
-    if (k.optimize == LBM_ARITH_EXACT && k.storage != LBM_STORE_NATIVE)
-        return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the fused form)");
-    if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
-        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
-        return fail(LBM_ERR_INVALID, "unknown kernel selection");
-    if (k.kernel == LBM_KERNEL_TILE) {
-        int64_t d[3];
-        if (k.storage != LBM_STORE_NATIVE) tile_dims<uint16_t>(d);
-        else if (k.precision == LBM_F64) tile_dims<double>(d);
-        else tile_dims<float>(d);
-        if (k.stride > d[0] || k.dim < d[0] || k.dim < d[1] || k.dim < d[2])
-            return fail(LBM_ERR_INVALID, "the tile kernel needs stride <= its x extent (" + std::to_string(d[0]) +
-                                             " cells) and dim >= that extent");
-    }
-    if (k.kernel == LBM_KERNEL_ROW && k.dim < 64) return fail(LBM_ERR_INVALID, "the row kernel needs dim >= 64");
-    if (k.kernel == LBM_KERNEL_VEC || k.kernel == LBM_KERNEL_VEC8) {
-        const int64_t lb = (k.kernel == LBM_KERNEL_VEC) ? 16 : 8;
-        const int64_t v = lb / ((k.storage != LBM_STORE_NATIVE) ? 2 : (k.precision == LBM_F64 ? 8 : 4));
-        if (k.stride < v || k.dim / v < 64)
-            return fail(LBM_ERR_INVALID, "the vector kernel needs stride >= V and dim / V >= 64 (V = lane bytes / "
-                                         "bytes per stored population)");
-    }
-    if (!(k.viscosity >= 0.0) || !std::isfinite(k.viscosity)) return fail(LBM_ERR_INVALID, "invalid viscosity");
-    if (!std::isfinite(k.velocity)) return fail(LBM_ERR_INVALID, "invalid velocity");
-    int64_t lws = 1;
-    for (int d = 0; d < 3; ++d) {
-        if (k.lws[d] < 0) return fail(LBM_ERR_INVALID, "negative work-group size");
-        lws *= (k.lws[d] > 0 ? k.lws[d] : 1);
-    }
-    if (lws > 1024) return fail(LBM_ERR_INVALID, "work-group size above 1024");
+    StepPlan plan;
+    const int prc = plan_from_config(k, &plan);
+    if (prc) return prc;
+    const int64_t n3 = plan.n3;
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess) {
@@ -121,26 +78,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
 
     lbm_ctx* c = new (std::nothrow) lbm_ctx();
     if (!c) return fail(LBM_ERR_MEMORY, "host allocation failed");
-    c->cfg = k;
-    c->n = k.dim;
-    c->log2n = ilog2(k.dim);
-    c->n3 = n3;
-    c->stride = k.stride;
-    c->tsize = (k.precision == LBM_F64) ? 8 : 4;
-    c->ssize = (k.storage == LBM_STORE_NATIVE) ? c->tsize : 2;
-    c->nu = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.viscosity)) : k.viscosity)
-                             : kernel_literal(k.viscosity, k.precision);
-    c->U = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.velocity)) : k.velocity)
-                            : kernel_literal(k.velocity, k.precision);
-    if (k.precision == LBM_F32) {
-        const float nu = float(c->nu);
-        const float tau = (3.0f * nu) + 0.5f;  // kernels.cl:61, folded in float
-        c->inv_tau = double(1.0f / tau);       // kernels.cl:62
-    } else {
-        const double tau = (3.0 * c->nu) + 0.5;
-        c->inv_tau = 1.0 / tau;
-    }
-    c->scale = (k.storage == LBM_STORE_NATIVE) ? 1.0 : shift_scale(c->U);
+    static_cast<StepPlan&>(*c) = plan;
 
     int rc = set_device(c);
     if (rc) {

@@ -101,10 +101,11 @@ struct SnapSlot {
     hipEvent_t packed = nullptr;
 };
 
-}  // namespace lbm_host
-
-struct lbm_ctx {
-    lbm_config cfg;
+// What one compute launch needs: the configuration, the derived constants, the
+// stream and the device buffers other than the two lattices.  A context holds
+// one; lbm_compute_noalloc builds one on the stack around caller buffers.
+struct StepPlan {
+    lbm_config cfg{};
     int32_t n = 0;
     int32_t log2n = 0;
     int64_t n3 = 0;
@@ -112,18 +113,23 @@ struct lbm_ctx {
     size_t tsize = 0;   // arithmetic type size
     size_t ssize = 0;   // storage element size
     hipStream_t stream = nullptr;
-    void* lat[2] = {nullptr, nullptr};  // lat[0] read by odd iterations, lat[1] by even
     void* rho = nullptr;
     void* u = nullptr;
+    double* partials = nullptr;  // 2 doubles per block of a compute launch (diagnostics)
+    void* force = nullptr;       // body force, 3 N^3 of the arithmetic type (null: none)
+    double nu = 0, inv_tau = 0, U = 0, scale = 1;
+};
+
+}  // namespace lbm_host
+
+struct lbm_ctx : lbm_host::StepPlan {
+    void* lat[2] = {nullptr, nullptr};  // lat[0] read by odd iterations, lat[1] by even
     int32_t* map = nullptr;      // cell flags, allocated on first read (ensure_map)
     bool map_valid = false;
-    double* partials = nullptr;
     int64_t partials_cap = 0;  // in blocks
     double* diag = nullptr;    // 2 doubles
     bool diag_valid = false;
     bool initialized = false;
-    double nu = 0, inv_tau = 0, U = 0, scale = 1;
-    void* force = nullptr;     // body force, 3 N^3 of the arithmetic type (null: none)
     lbm_host::EventLog events;
     // asynchronous snapshots: staging slots and their state machine
     hipStream_t copy_stream = nullptr;
@@ -255,29 +261,29 @@ template <class S> inline void tile_dims(int64_t* d) {
     d[1] = lbm::TileShape<S>::TY;
     d[2] = lbm::TileShape<S>::TZ;
 }
-inline int64_t tile_dim(const lbm_ctx* c, int k) {
+inline int64_t tile_dim(const StepPlan* c, int k) {
     int64_t d[3];
     if (c->ssize == 8) tile_dims<double>(d);
     else if (c->ssize == 4) tile_dims<float>(d);
     else tile_dims<uint16_t>(d);
     return d[k];
 }
-inline int64_t tile_b(const lbm_ctx* c) { return tile_dim(c, 0); }
-inline int64_t tile_ty(const lbm_ctx* c) { return tile_dim(c, 1); }
-inline int64_t tile_tz(const lbm_ctx* c) { return tile_dim(c, 2); }
+inline int64_t tile_b(const StepPlan* c) { return tile_dim(c, 0); }
+inline int64_t tile_ty(const StepPlan* c) { return tile_dim(c, 1); }
+inline int64_t tile_tz(const StepPlan* c) { return tile_dim(c, 2); }
 // The tile kernel needs the x extent to be whole stride blocks (stride <= B)
 // and the lattice to hold whole tiles.
-inline bool tile_possible(const lbm_ctx* c) {
+inline bool tile_possible(const StepPlan* c) {
     return c->stride <= tile_b(c) && c->n >= tile_b(c) && c->n >= tile_ty(c) && c->n >= tile_tz(c);
 }
 // AUTO picks it where a stride block of one population is at most 32 bytes:
 // measured on MI355X, there it beats the vector kernels, above it loses.
-inline bool tile_preferred(const lbm_ctx* c) { return c->stride * int64_t(c->ssize) <= 32 && tile_possible(c); }
+inline bool tile_preferred(const StepPlan* c) { return c->stride * int64_t(c->ssize) <= 32 && tile_possible(c); }
 
 // bytes per lane and population of the vector kernels
 inline int64_t lane_bytes(KernelKind k) { return k == K_VEC8 ? 8 : 16; }
 
-inline bool vec_possible(const lbm_ctx* c, KernelKind k) {
+inline bool vec_possible(const StepPlan* c, KernelKind k) {
     const int64_t v = lane_bytes(k) / int64_t(c->ssize);  // cells per lane
     return v >= 1 && c->stride >= v && c->n / v >= 64;
 }
@@ -287,9 +293,9 @@ inline bool vec_possible(const lbm_ctx* c, KernelKind k) {
 // four waves per SIMD in FP32 and 16-bit storage, and in FP64 (two cells per
 // lane) they were 1-2.5% slower than one cell per lane once the wavefront ends
 // came through LDS.
-inline KernelKind preferred_vec(const lbm_ctx*) { return K_VEC8; }
+inline KernelKind preferred_vec(const StepPlan*) { return K_VEC8; }
 
-inline KernelKind kernel_kind(const lbm_ctx* c) {
+inline KernelKind kernel_kind(const StepPlan* c) {
     switch (c->cfg.kernel) {
         case LBM_KERNEL_GENERIC: return K_GENERIC;
         case LBM_KERNEL_ROW: return K_ROW;
@@ -310,19 +316,19 @@ inline KernelKind kernel_kind(const lbm_ctx* c) {
 }
 
 // cells per lane of the selected kernel
-inline int64_t vec_width(const lbm_ctx* c) {
+inline int64_t vec_width(const StepPlan* c) {
     const KernelKind k = kernel_kind(c);
     return (k == K_VEC || k == K_VEC8) ? lane_bytes(k) / int64_t(c->ssize) : 1;
 }
 
 // threads per block and blocks per row of the row kernels
-inline int64_t row_block(const lbm_ctx* c) {
+inline int64_t row_block(const StepPlan* c) {
     const int64_t per_row = c->n / vec_width(c);
     return per_row < 256 ? per_row : 256;
 }
-inline int64_t row_gx(const lbm_ctx* c) { return (c->n / vec_width(c)) / row_block(c); }
+inline int64_t row_gx(const StepPlan* c) { return (c->n / vec_width(c)) / row_block(c); }
 
-inline int generic_block(const lbm_ctx* c) {
+inline int generic_block(const StepPlan* c) {
     int64_t b = 1;
     for (int d = 0; d < 3; ++d) b *= (c->cfg.lws[d] > 0 ? c->cfg.lws[d] : 1);
     if (c->cfg.lws[0] <= 0 && c->cfg.lws[1] <= 0 && c->cfg.lws[2] <= 0) b = 256;
@@ -330,7 +336,7 @@ inline int generic_block(const lbm_ctx* c) {
 }
 
 // Number of blocks of one compute launch (for the diagnostics partials).
-inline int64_t compute_blocks(const lbm_ctx* c) {
+inline int64_t compute_blocks(const StepPlan* c) {
     const KernelKind k = kernel_kind(c);
     if (k == K_TILE) return (c->n / tile_b(c)) * (c->n / tile_ty(c)) * (c->n / tile_tz(c));
     if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n);  // all rows
@@ -339,29 +345,103 @@ inline int64_t compute_blocks(const lbm_ctx* c) {
     return (c->n3 + b - 1) / b;
 }
 
-// Arithmetic form of the collision: the fused form for 16-bit storage always
-// and for native storage with -o (LBM_ARITH_FUSED); the default
-// (LBM_ARITH_OPENCL) evaluates kernels.cl's expressions the way ROCm's OpenCL
-// compiler does; LBM_ARITH_EXACT is the CPU oracle's form.
-inline int arith_of(const lbm_ctx* c) {
-    if (c->cfg.storage != LBM_STORE_NATIVE) return lbm::AR_FUSED;
+// Arithmetic form of the collision: the deviation form for 16-bit storage
+// always; for native storage the fused form with -o (LBM_ARITH_FUSED), the
+// default (LBM_ARITH_OPENCL) evaluating kernels.cl's expressions the way ROCm's
+// OpenCL compiler does, or LBM_ARITH_EXACT, the CPU oracle's form.
+inline int arith_of(const StepPlan* c) {
+    if (c->cfg.storage != LBM_STORE_NATIVE) return lbm::AR_DELTA;
     if (c->cfg.optimize == LBM_ARITH_FUSED) return lbm::AR_FUSED;
     if (c->cfg.optimize == LBM_ARITH_EXACT) return lbm::AR_EXACT;
     return lbm::AR_OPENCL;
 }
 
+// Check a configuration and derive the step constants into `p` (no device
+// access): the validation of lbm_create, shared with lbm_compute_noalloc.
+inline int plan_from_config(const lbm_config& k, StepPlan* p) {
+    if (k.struct_size != sizeof(lbm_config))
+        return fail(LBM_ERR_INVALID, "lbm_config.struct_size is " + std::to_string(k.struct_size) +
+                                         ", this library's lbm_config has " + std::to_string(sizeof(lbm_config)) +
+                                         " bytes (set it to sizeof(lbm_config), e.g. with LBM_CONFIG_INIT)");
+    if (!is_pow2(k.dim)) return fail(LBM_ERR_INVALID, "dim must be a positive power of two");
+    if (k.dim > 1024) return fail(LBM_ERR_INVALID, "dim above 1024 is not supported");
+    const int64_t n3 = int64_t(k.dim) * k.dim * k.dim;
+    if (!is_pow2(k.stride) || k.stride > n3)
+        return fail(LBM_ERR_INVALID, "stride must be a power of two in [1, dim^3]");
+    if (k.precision != LBM_F32 && k.precision != LBM_F64) return fail(LBM_ERR_INVALID, "unknown precision");
+    if (k.storage != LBM_STORE_NATIVE && k.storage != LBM_STORE_FP16_SHIFT && k.storage != LBM_STORE_BF16_SHIFT)
+        return fail(LBM_ERR_INVALID, "unknown storage");
+    if (k.storage != LBM_STORE_NATIVE && k.precision != LBM_F32)
+        return fail(LBM_ERR_INVALID, "shifted 16-bit storage computes in FP32");
+    if (k.optimize != LBM_ARITH_REFERENCE && k.optimize != LBM_ARITH_FUSED && k.optimize != LBM_ARITH_EXACT)
+        return fail(LBM_ERR_INVALID,
+                    "optimize must be LBM_ARITH_REFERENCE (LBM_ARITH_OPENCL), LBM_ARITH_FUSED or LBM_ARITH_EXACT");
+    if (k.optimize == LBM_ARITH_EXACT && k.storage != LBM_STORE_NATIVE)
+        return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the fused form)");
+    if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
+        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
+        return fail(LBM_ERR_INVALID, "unknown kernel selection");
+    if (k.kernel == LBM_KERNEL_TILE) {
+        int64_t d[3];
+        if (k.storage != LBM_STORE_NATIVE) tile_dims<uint16_t>(d);
+        else if (k.precision == LBM_F64) tile_dims<double>(d);
+        else tile_dims<float>(d);
+        if (k.stride > d[0] || k.dim < d[0] || k.dim < d[1] || k.dim < d[2])
+            return fail(LBM_ERR_INVALID, "the tile kernel needs stride <= its x extent (" + std::to_string(d[0]) +
+                                             " cells) and dim >= that extent");
+    }
+    if (k.kernel == LBM_KERNEL_ROW && k.dim < 64) return fail(LBM_ERR_INVALID, "the row kernel needs dim >= 64");
+    if (k.kernel == LBM_KERNEL_VEC || k.kernel == LBM_KERNEL_VEC8) {
+        const int64_t lb = (k.kernel == LBM_KERNEL_VEC) ? 16 : 8;
+        const int64_t v = lb / ((k.storage != LBM_STORE_NATIVE) ? 2 : (k.precision == LBM_F64 ? 8 : 4));
+        if (k.stride < v || k.dim / v < 64)
+            return fail(LBM_ERR_INVALID, "the vector kernel needs stride >= V and dim / V >= 64 (V = lane bytes / "
+                                         "bytes per stored population)");
+    }
+    if (!(k.viscosity >= 0.0) || !std::isfinite(k.viscosity)) return fail(LBM_ERR_INVALID, "invalid viscosity");
+    if (!std::isfinite(k.velocity)) return fail(LBM_ERR_INVALID, "invalid velocity");
+    int64_t lws = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (k.lws[d] < 0) return fail(LBM_ERR_INVALID, "negative work-group size");
+        lws *= (k.lws[d] > 0 ? k.lws[d] : 1);
+    }
+    if (lws > 1024) return fail(LBM_ERR_INVALID, "work-group size above 1024");
+
+    p->cfg = k;
+    p->n = k.dim;
+    p->log2n = ilog2(k.dim);
+    p->n3 = n3;
+    p->stride = k.stride;
+    p->tsize = (k.precision == LBM_F64) ? 8 : 4;
+    p->ssize = (k.storage == LBM_STORE_NATIVE) ? p->tsize : 2;
+    p->nu = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.viscosity)) : k.viscosity)
+                             : kernel_literal(k.viscosity, k.precision);
+    p->U = k.raw_parameters ? (k.precision == LBM_F32 ? double(float(k.velocity)) : k.velocity)
+                            : kernel_literal(k.velocity, k.precision);
+    if (k.precision == LBM_F32) {
+        const float nu = float(p->nu);
+        const float tau = (3.0f * nu) + 0.5f;  // kernels.cl:61, folded in float
+        p->inv_tau = double(1.0f / tau);       // kernels.cl:62
+    } else {
+        const double tau = (3.0 * p->nu) + 0.5;
+        p->inv_tau = 1.0 / tau;
+    }
+    p->scale = (k.storage == LBM_STORE_NATIVE) ? 1.0 : shift_scale(p->U);
+    return LBM_OK;
+}
+
 // Compute launches in the OpenCL arithmetic form, defined in
 // lbm_opencl_arith.hip (built with OpenCL's FP32 division precision).
-int launch_compute_opencl_f32(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+int launch_compute_opencl_f32(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                               hipEvent_t e1);
-int launch_compute_opencl_f64(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+int launch_compute_opencl_f64(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                               hipEvent_t e1);
 
 // Launch on the context's stream; with events, through hipExtLaunchKernel,
 // which stamps the kernel's own start and end into them (no separate marker
 // packets between launches, like the reference's profiling-queue events).
 template <class K, class A>
-inline void launch_on(lbm_ctx* c, K kernel, dim3 grid, dim3 block, hipEvent_t start, hipEvent_t stop, const A& a) {
+inline void launch_on(StepPlan* c, K kernel, dim3 grid, dim3 block, hipEvent_t start, hipEvent_t stop, const A& a) {
     if (start)
         hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, start, stop, 0, a);
     else
@@ -369,7 +449,7 @@ inline void launch_on(lbm_ctx* c, K kernel, dim3 grid, dim3 block, hipEvent_t st
 }
 
 template <class T, int KIND, int ARITH>
-inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                                hipEvent_t e1) {
     lbm::StepArgs<T> a{};
     a.src = src;
@@ -393,7 +473,11 @@ inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int updat
     {
         // lid-layer initial populations, same expression as the initialize kernel
         const T U = T(c->U);
-        if constexpr (ARITH == lbm::AR_OPENCL) {
+        if constexpr (ARITH == lbm::AR_DELTA) {
+            float d[lbm::Q];
+            lbm::delta_lid_initial(float(c->U), d);
+            for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = T(d[i]);
+        } else if constexpr (ARITH == lbm::AR_OPENCL) {
             const T u2 = lbm::u2_contracted<T>(U, T(0), T(0));
             for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq_contracted<T>(i, T(1), U, T(0), T(0), u2);
         } else {
@@ -429,10 +513,10 @@ inline int launch_compute_impl(lbm_ctx* c, const void* src, void* dst, int updat
 
 // e0/e1: events stamped with the kernel's start and end (null: none)
 template <class T, int KIND>
-inline int launch_compute(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0 = nullptr,
+inline int launch_compute(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr) {
     if constexpr (KIND != lbm::ST_NATIVE) {
-        return launch_compute_impl<T, KIND, lbm::AR_FUSED>(c, src, dst, update_macro, diag, e0, e1);
+        return launch_compute_impl<T, KIND, lbm::AR_DELTA>(c, src, dst, update_macro, diag, e0, e1);
     } else {
         const int ar = arith_of(c);
         if (ar == lbm::AR_FUSED)
@@ -460,6 +544,11 @@ inline int launch_initialize(lbm_ctx* c) {
     a.scale = float(c->scale);
     a.inv_scale = float(1.0 / c->scale);
     a.contracted = arith_of(c) == lbm::AR_OPENCL;
+    if constexpr (KIND != lbm::ST_NATIVE) {
+        float d[lbm::Q];
+        lbm::delta_lid_initial(float(c->U), d);
+        for (int i = 0; i < lbm::Q; ++i) a.lid_delta[i] = T(d[i]);
+    }
     const int64_t nb = (c->n3 + 255) / 256;
     hipLaunchKernelGGL((lbm::lbm_initialize_kernel<T, KIND>), dim3(unsigned(nb)), dim3(256), 0, c->stream, a);
     LBM_HIP_CHECK(hipGetLastError(), "initialize launch");

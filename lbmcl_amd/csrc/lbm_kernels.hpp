@@ -58,6 +58,9 @@ template <> struct Codec<float, ST_FP16> {
     }
     __device__ __forceinline__ float load(const S* p, float w) const { return decode(*p, w); }
     __device__ __forceinline__ void store(S* p, float w, float v) const { __builtin_nontemporal_store(encode(v, w), p); }
+    // deviation form: d = f - w_i itself, h = RN16(d * scale), d = h * (1/scale) (both products exact)
+    __device__ __forceinline__ float decode_delta(S s) const { return float(s) * inv_scale; }
+    __device__ __forceinline__ S encode_delta(float d) const { return _Float16(d * scale); }
 };
 
 template <> struct Codec<float, ST_BF16> {
@@ -73,6 +76,8 @@ template <> struct Codec<float, ST_BF16> {
     }
     __device__ __forceinline__ float load(const S* p, float w) const { return decode(*p, w); }
     __device__ __forceinline__ void store(S* p, float w, float v) const { __builtin_nontemporal_store(encode(v, w), p); }
+    __device__ __forceinline__ float decode_delta(S s) const { return __uint_as_float(uint32_t(s) << 16) * inv_scale; }
+    __device__ __forceinline__ S encode_delta(float d) const { return __builtin_bit_cast(uint16_t, __bf16(d * scale)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -170,7 +175,7 @@ template <class T> __host__ __device__ __forceinline__ T feq_contracted(int i, T
 // ROCm's OpenCL compiler evaluates them (contracted as above, FP32 division to
 // OpenCL's 2.5 ulp; those kernels live in lbm_opencl_arith.hip, the one
 // translation unit built with that division precision).
-enum Arith { AR_EXACT = 0, AR_FUSED = 1, AR_OPENCL = 2 };
+enum Arith { AR_EXACT = 0, AR_FUSED = 1, AR_OPENCL = 2, AR_DELTA = 3 };
 
 // Lid fix-up and macroscopic values of a non-wall cell (kernels.cl:343-379):
 // on return f holds the fixed-up populations, rho and u the moments.
@@ -315,12 +320,110 @@ __device__ __forceinline__ void add_force(int32_t t, T (&f)[Q], T gx, T gy, T gz
     for (int i = 0; i < Q; ++i) f[i] = f[i] + (T(3) * w_of<T>(i)) * edot<T>(i, gx, gy, gz);
 }
 
+// ---------------------------------------------------------------------------
+// Deviation ("delta") form of the local step, the arithmetic of the 16-bit
+// storage forms (no counterpart in the reference).  The populations stay
+// deviations d_i = f_i - w_i from load to store -- what the 16-bit lattice
+// holds, scaled -- so no value near w_i is ever rounded in FP32.  (Rounding
+// f_i itself, whose low bits all cells of a smooth flow share with w_i,
+// gives correlated errors: with 16-bit-decoded inputs the total mass then
+// drifted by ~3e-7 of itself per step.)  Moments:
+//   drho = d_0 + ... + d_18 (rho = 1 + drho),  j = sum e_i d_i (sum e_i w_i = 0);
+// the lid fix-up and bounce-back move d_i as the reference moves f_i; BGK
+// f' = (1 - w) f + w rho w_i T_i becomes, with tau_i = T_i - 1,
+//   d_i' = (w w_i) fma(rho, tau_i, drho) + (1 - w) d_i,
+//   tau_i = Km1 + e (3 + 4.5 e),  Km1 = -1.5 u2,  e = e_i . u,
+// tau shared between opposite populations as A +- 3e, A = 4.5 e e + Km1; the
+// lid sets d_i = w_i fma(rho, tau_i, drho).  The CPU oracle evaluates the same
+// operations in the same order.
+// ---------------------------------------------------------------------------
+// d_i <- fma(cw_i, fma(rho, tau_i, drho), keep * d_i); cw_i = c0, c1, c7 by |e_i|^2
+__host__ __device__ __forceinline__ void delta_relax(float rho, float drho, float ux, float uy, float uz, float c0,
+                                                     float c1, float c7, float keep, float (&d)[Q]) {
+    const float u2 = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
+    const float Km1 = -1.5f * u2;
+    constexpr int PI[9] = {1, 2, 6, 7, 10, 15, 16, 11, 12};  // e_i
+    constexpr int PO[9] = {3, 4, 5, 9, 8, 13, 14, 17, 18};   // -e_i
+    d[0] = __builtin_fmaf(c0, __builtin_fmaf(rho, Km1, drho), keep * d[0]);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int i = PI[p], o = PO[p];
+        const float e = edot<float>(i, ux, uy, uz);
+        const float A = __builtin_fmaf(4.5f * e, e, Km1);
+        const float B = 3.0f * e;
+        const float c = (i <= 6) ? c1 : c7;
+        d[i] = __builtin_fmaf(c, __builtin_fmaf(rho, A + B, drho), keep * d[i]);
+        d[o] = __builtin_fmaf(c, __builtin_fmaf(rho, A - B, drho), keep * d[o]);
+    }
+}
+
+// Initial deviations of a non-wall cell with u = (U, 0, 0) (FRONT cells):
+// w_i tau_i at rho = 1; other cells start at d = 0.
+__host__ __device__ __forceinline__ void delta_lid_initial(float U, float (&d)[Q]) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = 0.0f;
+    delta_relax(1.0f, 0.0f, U, 0.0f, 0.0f, W<float>::w0, W<float>::w1, W<float>::w7, 0.0f, d);
+}
+
+template <class T>
+__device__ __forceinline__ void collide_delta(int32_t t, T (&d)[Q], T& rho, T& ux, T& uy, T& uz,
+                                              const StepConsts<T>& k) {
+    static_assert(sizeof(T) == 4, "the deviation form computes in FP32");
+    const bool moving = is_moving(t);
+    if (moving) {  // kernels.cl:343-349
+        d[5] = d[6];
+        d[11] = d[17];
+        d[12] = d[18];
+        d[13] = d[15];
+        d[14] = d[16];
+    }
+    T drho = d[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) drho = drho + d[i];
+    rho = T(1) + drho;
+    if (moving) {  // kernels.cl:362-365
+        ux = k.U;
+        uy = T(0);
+        uz = T(0);
+    } else {  // kernels.cl:376-378 on the deviations
+        const T jx = ((((d[1] + d[7]) + d[10]) + d[11]) + d[15]) - ((((d[3] + d[8]) + d[9]) + d[13]) + d[17]);
+        const T jy = ((((d[2] + d[7]) + d[8]) + d[12]) + d[16]) - ((((d[4] + d[9]) + d[10]) + d[14]) + d[18]);
+        const T jz = ((((d[6] + d[15]) + d[16]) + d[17]) + d[18]) - ((((d[5] + d[11]) + d[12]) + d[13]) + d[14]);
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = jz / rho;
+    }
+    if (!moving && t != FLUID) {
+        if (is_bounceback(t)) bounce_back<T>(d);
+        return;  // CORNER: pure propagation
+    }
+    if (moving)
+        delta_relax(rho, drho, ux, uy, uz, W<T>::w0, W<T>::w1, W<T>::w7, T(0), d);
+    else
+        delta_relax(rho, drho, ux, uy, uz, k.inv_tau * W<T>::w0, k.inv_tau * W<T>::w1, k.inv_tau * W<T>::w7, k.om1,
+                    d);
+}
+
 template <class T, int ARITH>
 __device__ __forceinline__ void collide_any(int32_t t, T (&f)[Q], T& rho, T& ux, T& uy, T& uz,
                                             const StepConsts<T>& k) {
     if constexpr (ARITH == AR_FUSED) collide_fast<T>(t, f, rho, ux, uy, uz, k);
     else if constexpr (ARITH == AR_OPENCL) collide_opencl<T>(t, f, rho, ux, uy, uz, k);
+    else if constexpr (ARITH == AR_DELTA) collide_delta<T>(t, f, rho, ux, uy, uz, k);
     else collide<T>(t, f, rho, ux, uy, uz, k);
+}
+
+// Storage <-> collision values: the population, or its deviation from w_i in
+// the deviation form.
+template <int ARITH, class C, class S, class T>
+__device__ __forceinline__ T value_in(const C& codec, S s, T w) {
+    if constexpr (ARITH == AR_DELTA) return codec.decode_delta(s);
+    else return codec.decode(s, w);
+}
+template <int ARITH, class C, class T>
+__device__ __forceinline__ typename C::S value_out(const C& codec, T v, T w) {
+    if constexpr (ARITH == AR_DELTA) return codec.encode_delta(v);
+    else return codec.encode(v, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,7 +557,7 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
             const int64_t c0 = int64_t(n) * (r - ey - int64_t(n) * ez);  // first cell of the source row
             const int64_t base = 19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride;
             const uint32_t off = ex == 1 ? v_m : (ex == -1 ? v_p : v_0);
-            f[i] = codec.load(src + base + off, w_of<T>(i));
+            f[i] = value_in<ARITH>(codec, src[base + off], w_of<T>(i));
         }
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         if (a.force) {
@@ -464,7 +567,8 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
         const int64_t c0 = int64_t(n) * r;
         const int64_t obase = 19 * c0 - 18 * (c0 & smask);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) codec.store(dst + obase + int64_t(i) * a.stride + v_0, w_of<T>(i), f[i]);
+        for (int i = 0; i < Q; ++i)
+            __builtin_nontemporal_store(value_out<ARITH>(codec, f[i], w_of<T>(i)), dst + obase + int64_t(i) * a.stride + v_0);
         if (a.update_macro && is_store_macro(t)) {
             const int64_t c = int64_t(x) + int64_t(n) * r;
             a.rho[c] = rho;
@@ -527,12 +631,14 @@ template <class S> __device__ __forceinline__ S lane_shift(S v, S edge, bool fro
 // Initial pull-form value of population i at the wall cell x = 0 (side -1) or
 // x = N-1 (side +1) of row (y, z): F0 of the destination (x + e_x, y + e_y,
 // z + e_z), i.e. NaN unless the destination is a fluid-side cell.
-template <class T>
+// In the deviation form (ARITH == AR_DELTA) the value is F0 - w_i: lid_eq then
+// holds the lid-layer deviations and the rest value is 0.
+template <class T, int ARITH = AR_EXACT>
 __device__ __forceinline__ T wall_p0(int i, int side, int32_t y, int32_t z, int32_t n, const T* lid_eq) {
     const int32_t dy = y + ey_of(i), dz = z + ez_of(i);
     const bool into = (side < 0) ? (ex_of(i) == 1) : (ex_of(i) == -1);
     if (!into || dy < 1 || dy > n - 2 || dz < 1 || dz > n - 2) return T(__builtin_nan(""));
-    return dz == n - 2 ? lid_eq[i] : w_of<T>(i);
+    return dz == n - 2 ? lid_eq[i] : (ARITH == AR_DELTA ? T(0) : w_of<T>(i));
 }
 
 // Occupancy target of the vector kernel: the waves per SIMD its register
@@ -652,7 +758,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         const int32_t t = finish_type(x_flags(x0 + j, n) | rf);
         T f[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = codec.decode(g[i][j], w_of<T>(i));
+        for (int i = 0; i < Q; ++i) f[i] = value_in<ARITH>(codec, g[i][j], w_of<T>(i));
         T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
         if (a.force) {
@@ -677,12 +783,12 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
             const bool wr = (j == V - 1) && (x0 + V - 1 == n - 1);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                if (wl) f[i] = wall_p0<T>(i, -1, y, z, n, a.lid_eq);
-                if (wr) f[i] = wall_p0<T>(i, +1, y, z, n, a.lid_eq);
+                if (wl) f[i] = wall_p0<T, ARITH>(i, -1, y, z, n, a.lid_eq);
+                if (wr) f[i] = wall_p0<T, ARITH>(i, +1, y, z, n, a.lid_eq);
             }
         }
 #pragma unroll
-        for (int i = 0; i < Q; ++i) g[i][j] = codec.encode(f[i], w_of<T>(i));
+        for (int i = 0; i < Q; ++i) g[i][j] = value_out<ARITH>(codec, f[i], w_of<T>(i));
     }
 
     const int64_t oc0 = int64_t(n) * r;
@@ -876,7 +982,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
             if (ex_of(i) == 1 && xs < 0) v = halo_l[r][xp_slot(i)];
             else if (ex_of(i) == -1 && xs >= B) v = halo_r[r][xm_slot(i)];
             else v = span[r][csoa(xs, i)];
-            f[i] = codec.decode(v, w_of<T>(i));
+            f[i] = value_in<ARITH>(codec, v, w_of<T>(i));
         }
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         const int64_t c = int64_t(x) + int64_t(n) * (int64_t(y) + int64_t(n) * z);
@@ -889,7 +995,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
         }
     } else if (row_ok) {  // the row's wall cells keep their initial values
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = wall_p0<T>(i, x == 0 ? -1 : +1, y, z, n, a.lid_eq);
+        for (int i = 0; i < Q; ++i) f[i] = wall_p0<T, ARITH>(i, x == 0 ? -1 : +1, y, z, n, a.lid_eq);
     }
     __syncthreads();  // every gather is done: the spans are reused for the output
 
@@ -897,7 +1003,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
     const int ro = ty + TY * tz;
     if (row_ok) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) span[ro][csoa(xl, i)] = codec.encode(f[i], w_of<T>(i));
+        for (int i = 0; i < Q; ++i) span[ro][csoa(xl, i)] = value_out<ARITH>(codec, f[i], w_of<T>(i));
     }
     __syncthreads();
     constexpr int PERO = (TY * TZ * CH + NT - 1) / NT;
@@ -950,12 +1056,13 @@ __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int64_t s = c - (ex_of(i) + int64_t(n) * (ey_of(i) + int64_t(n) * ez_of(i)));
-            f[i] = codec.load(src + slot(s, i, a.stride), w_of<T>(i));
+            f[i] = value_in<ARITH>(codec, src[slot(s, i, a.stride)], w_of<T>(i));
         }
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) codec.store(dst + slot(c, i, a.stride), w_of<T>(i), f[i]);
+        for (int i = 0; i < Q; ++i)
+            __builtin_nontemporal_store(value_out<ARITH>(codec, f[i], w_of<T>(i)), dst + slot(c, i, a.stride));
         if (a.update_macro && is_store_macro(t)) {
             a.rho[c] = rho;
             a.u[c] = ux;
@@ -992,6 +1099,7 @@ template <class T> struct InitArgs {
     T U;
     float scale, inv_scale;
     int32_t contracted;  // nonzero: the equilibrium as ROCm's OpenCL compiler evaluates it
+    T lid_delta[Q];      // 16-bit storage (deviation form): F0 - w_i of FRONT cells
 };
 
 template <class T, int KIND>
@@ -1020,7 +1128,9 @@ __global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
         T v = qnan;
         if (dx >= 0 && dy >= 0 && dz >= 0 && dx < n && dy < n && dz < n) {
             const int32_t td = cell_type(dx, dy, dz, n);
-            if (!is_wall(td)) {
+            if (!is_wall(td) && KIND != ST_NATIVE) {
+                v = is_moving_init(td) ? a.lid_delta[i] : T(0);  // deviation form
+            } else if (!is_wall(td)) {
                 // kernels.cl:292-308 with rho = 1, u = (U or 0, 0, 0)
                 const T ux = is_moving_init(td) ? a.U : T(0);
                 if (a.contracted) {
@@ -1032,8 +1142,14 @@ __global__ __launch_bounds__(256) void lbm_initialize_kernel(InitArgs<T> a) {
             }
         }
         const int64_t s = slot(c, i, a.stride);
-        codec.store(pa + s, w_of<T>(i), v);
-        codec.store(pb + s, w_of<T>(i), v);
+        if constexpr (KIND != ST_NATIVE) {
+            const S h = codec.encode_delta(v);
+            __builtin_nontemporal_store(h, pa + s);
+            __builtin_nontemporal_store(h, pb + s);
+        } else {
+            codec.store(pa + s, w_of<T>(i), v);
+            codec.store(pb + s, w_of<T>(i), v);
+        }
     }
 }
 

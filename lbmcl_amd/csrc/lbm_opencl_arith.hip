@@ -10,12 +10,12 @@
 
 namespace lbm_host {
 
-int launch_compute_opencl_f32(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+int launch_compute_opencl_f32(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                               hipEvent_t e1) {
     return launch_compute_impl<float, lbm::ST_NATIVE, lbm::AR_OPENCL>(c, src, dst, update_macro, diag, e0, e1);
 }
 
-int launch_compute_opencl_f64(lbm_ctx* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
+int launch_compute_opencl_f64(StepPlan* c, const void* src, void* dst, int update_macro, int diag, hipEvent_t e0,
                               hipEvent_t e1) {
     return launch_compute_impl<double, lbm::ST_NATIVE, lbm::AR_OPENCL>(c, src, dst, update_macro, diag, e0, e1);
 }

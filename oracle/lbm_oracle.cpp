@@ -210,6 +210,102 @@ inline void collide_fused(const Consts<T>& k, int32_t t, T* f, T rho, T ux, T uy
     }
 }
 
+// Deviation ("delta") form of the step, used with 16-bit storage (no
+// counterpart in the reference).  The populations are held as d_i = f_i - w_i
+// from load to store -- the stored 16-bit values are scaled d_i -- so no value
+// near w_i is ever rounded in FP32: the sums, the equilibrium and the
+// relaxation work on small deviations.  The moments are
+//   drho = d_0 + ... + d_18 (rho = 1 + drho),  j = sum e_i d_i  (sum e_i w_i = 0),
+// the lid fix-up and bounce-back move d_i as the reference moves f_i, and the
+// BGK update f' = (1 - w) f + w rho w_i T_i is written for the deviations:
+//   d_i' = (w w_i) (rho tau_i + drho) + (1 - w) d_i,   tau_i = T_i - 1,
+//   tau_i = Km1 + e (3 + 4.5 e),  Km1 = -1.5 u2,  e = e_i . u,
+// (rho tau_i + drho = rho T_i - 1), with tau shared between opposite
+// populations as A +- 3e, A = 4.5 e e + Km1; the lid sets d_i = w_i (rho tau_i +
+// drho).  Each std::fma is one IEEE rounding; the GPU kernels evaluate the
+// same operations in the same order.
+inline void delta_eq_all(const Consts<float>& k, float rho, float drho, float ux, float uy, float uz, float cw0,
+                         float cw1, float cw7, float keep, float* d) {
+    // d_i <- fma(cw_i, fma(rho, tau_i, drho), keep * d_i)
+    const float u2 = std::fma(uz, uz, std::fma(uy, uy, ux * ux));
+    const float Km1 = -1.5f * u2;
+    const int PI[9] = {1, 2, 6, 7, 10, 15, 16, 11, 12};
+    const int PO[9] = {3, 4, 5, 9, 8, 13, 14, 17, 18};
+    d[0] = std::fma(cw0, std::fma(rho, Km1, drho), keep * d[0]);
+    for (int p = 0; p < 9; ++p) {
+        const int i = PI[p], o = PO[p];
+        const float e = ((ux * float(EX[i])) + (uy * float(EY[i]))) + (uz * float(EZ[i]));
+        const float A = std::fma(4.5f * e, e, Km1);
+        const float B = 3.0f * e;
+        const float c = (i <= 6) ? cw1 : cw7;
+        d[i] = std::fma(c, std::fma(rho, A + B, drho), keep * d[i]);
+        d[o] = std::fma(c, std::fma(rho, A - B, drho), keep * d[o]);
+    }
+    (void)k;
+}
+
+// The local step in deviation form; on entry d holds the gathered deviations,
+// on return the ones to store; rho (= 1 + drho) and u as the reference defines them.
+inline void collide_delta(const Consts<float>& k, int32_t t, float* d, float* rho_out, float* ux_out,
+                          float* uy_out, float* uz_out, const float* force) {
+    const bool moving = is_moving(t);
+    if (moving) {  // kernels.cl:343-349
+        d[5] = d[6];
+        d[11] = d[17];
+        d[12] = d[18];
+        d[13] = d[15];
+        d[14] = d[16];
+    }
+    const float drho = moment_rho(d);
+    const float rho = 1.0f + drho;
+    float ux, uy, uz;
+    if (moving) {  // kernels.cl:362-365
+        ux = k.U;
+        uy = 0.0f;
+        uz = 0.0f;
+    } else {
+        moment_u(d, rho, ux, uy, uz);
+    }
+    *rho_out = rho;
+    *ux_out = ux;
+    *uy_out = uy;
+    *uz_out = uz;
+    if (!moving && !is_fluid(t)) {
+        if (is_bounceback(t)) {
+            const int half[9] = {1, 2, 5, 7, 8, 11, 12, 13, 14};
+            for (int h : half) {
+                const float tmp = d[h];
+                d[h] = d[OPP[h]];
+                d[OPP[h]] = tmp;
+            }
+        }
+        return;
+    }
+    if (moving) {
+        delta_eq_all(k, rho, drho, ux, uy, uz, k.w[0], k.w[1], k.w[7], 0.0f, d);
+    } else {
+        const float om = k.inv_tau;
+        delta_eq_all(k, rho, drho, ux, uy, uz, om * k.w[0], om * k.w[1], om * k.w[7], k.om1, d);
+        if (force) {
+            for (int i = 0; i < Q; ++i) {
+                const float eg = ((force[0] * float(EX[i])) + (force[1] * float(EY[i]))) + (force[2] * float(EZ[i]));
+                d[i] = d[i] + (3.0f * k.w[i]) * eg;
+            }
+        }
+    }
+}
+
+// Initial deviations of a cell: w_i tau_i at rho = 1 (drho = 0) and
+// u = (U, 0, 0) on FRONT cells, 0 elsewhere; NaN for walls.
+inline void init_cell_delta(const Consts<float>& k, int32_t t, float* d) {
+    if (is_wall(t)) {
+        for (int i = 0; i < Q; ++i) d[i] = std::numeric_limits<float>::quiet_NaN();
+        return;
+    }
+    for (int i = 0; i < Q; ++i) d[i] = 0.0f;
+    if (is_moving_init(t)) delta_eq_all(k, 1.0f, 0.0f, k.U, 0.0f, 0.0f, k.w[0], k.w[1], k.w[7], 0.0f, d);
+}
+
 // The local part of the compute kernel (kernels.cl:343-418): lid fix-up,
 // moments, optional macro store, boundary conditions and BGK collision.
 // `f` holds the 19 populations that arrived at the cell; on return it holds the
@@ -464,6 +560,16 @@ struct Codec {
         const float v = (kind == STORE_FP16) ? f16_to_f32(h) : bf16_to_f32(h);
         return T(std::fma(v, inv_scale, float(w[i])));  // = w + v / scale: the product is exact
     }
+    // 16-bit storage holds the deviation d = f - w_i scaled: h = RN16(d * scale),
+    // d = h * (1/scale); both products are exact (scale is a power of two)
+    inline float load_delta(const void* base, int64_t s) const {
+        const uint16_t h = static_cast<const uint16_t*>(base)[s];
+        return ((kind == STORE_FP16) ? f16_to_f32(h) : bf16_to_f32(h)) * inv_scale;
+    }
+    inline void store_delta(void* base, int64_t s, float d) const {
+        const float v = d * scale;
+        static_cast<uint16_t*>(base)[s] = (kind == STORE_FP16) ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+    }
     inline void store(void* base, int64_t s, int i, T f) const {
         if (kind == STORE_NATIVE) {
             static_cast<T*>(base)[s] = f;
@@ -501,10 +607,19 @@ void pull_initialize(const Params& p, int storage, void* Pa, void* Pb, T* rho, T
                 u[id] = st ? ux : nan_v<T>();
                 u[N3 + id] = st ? uy : nan_v<T>();
                 u[2 * N3 + id] = st ? uz : nan_v<T>();
+                float d[Q] = {};
+                if constexpr (std::is_same<T, float>::value) {
+                    if (storage != STORE_NATIVE) init_cell_delta(k, t, d);
+                }
                 for (int i = 0; i < Q; ++i) {
                     const int64_t sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
                     if (sx < 0 || sy < 0 || sz < 0 || sx >= N || sy >= N || sz >= N) continue;
                     const int64_t s = slot(sx + sy * N + sz * N * N, i, S);
+                    if (storage != STORE_NATIVE) {
+                        cd.store_delta(Pa, s, d[i]);
+                        cd.store_delta(Pb, s, d[i]);
+                        continue;
+                    }
                     cd.store(Pa, s, i, f[i]);
                     cd.store(Pb, s, i, f[i]);
                 }
@@ -518,6 +633,11 @@ void pull_initialize(const Params& p, int storage, void* Pa, void* Pb, T* rho, T
                     const int64_t dx = x + EX[i], dy = y + EY[i], dz = z + EZ[i];
                     if (dx >= 0 && dy >= 0 && dz >= 0 && dx < N && dy < N && dz < N) continue;
                     const int64_t s = slot(x + y * N + z * N * N, i, S);
+                    if (storage != STORE_NATIVE) {
+                        cd.store_delta(Pa, s, std::numeric_limits<float>::quiet_NaN());
+                        cd.store_delta(Pb, s, std::numeric_limits<float>::quiet_NaN());
+                        continue;
+                    }
                     cd.store(Pa, s, i, nan_v<T>());
                     cd.store(Pb, s, i, nan_v<T>());
                 }
@@ -539,10 +659,6 @@ void pull_compute(const Params& p, int storage, const void* Pin, void* Pout, T* 
                 const int32_t t = cell_type(N, x, y, z);
                 if (is_wall(t)) continue;
                 T f[Q];
-                for (int i = 0; i < Q; ++i) {
-                    const int64_t sid = (x - EX[i]) + (y - EY[i]) * N + (z - EZ[i]) * N * N;
-                    f[i] = cd.load(Pin, slot(sid, i, S), i);
-                }
                 T r, ux, uy, uz, g[3];
                 const T* gp = nullptr;
                 if (force) {
@@ -551,7 +667,22 @@ void pull_compute(const Params& p, int storage, const void* Pin, void* Pout, T* 
                     g[2] = force[2 * N3 + id];
                     gp = g;
                 }
-                collide(k, t, f, &r, &ux, &uy, &uz, gp);
+                if constexpr (std::is_same<T, float>::value) {
+                    if (storage != STORE_NATIVE) {  // deviation form
+                        for (int i = 0; i < Q; ++i) {
+                            const int64_t sid = (x - EX[i]) + (y - EY[i]) * N + (z - EZ[i]) * N * N;
+                            f[i] = cd.load_delta(Pin, slot(sid, i, S));
+                        }
+                        collide_delta(k, t, f, &r, &ux, &uy, &uz, gp);
+                    }
+                }
+                if (storage == STORE_NATIVE) {
+                    for (int i = 0; i < Q; ++i) {
+                        const int64_t sid = (x - EX[i]) + (y - EY[i]) * N + (z - EZ[i]) * N * N;
+                        f[i] = cd.load(Pin, slot(sid, i, S), i);
+                    }
+                    collide(k, t, f, &r, &ux, &uy, &uz, gp);
+                }
                 if (is_store_macro(t)) {
                     if (update_macro) {
                         rho[id] = r;
@@ -564,7 +695,8 @@ void pull_compute(const Params& p, int storage, const void* Pin, void* Pout, T* 
                     ke += 0.5 * dr * (((dx * dx) + (dy * dy)) + (dz * dz));
                 }
                 for (int i = 0; i < Q; ++i) {
-                    cd.store(Pout, slot(id, i, S), i, f[i]);
+                    if (storage != STORE_NATIVE) cd.store_delta(Pout, slot(id, i, S), float(f[i]));
+                    else cd.store(Pout, slot(id, i, S), i, f[i]);
                     if (writes) {
 #pragma omp atomic
                         ++writes[slot(id, i, S)];
