@@ -97,6 +97,26 @@ def copy_bandwidth(nbytes=4 << 30, reps=5):
     return {"bytes": nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
 
 
+def access_pattern_bandwidth(stride, reps=6):
+    """The FP32 step's data movement at 512^3 without its arithmetic (membw.hip
+    pattern_kernel): algorithmic GB/s (152 B per wet cell) at full occupancy and
+    at two waves per SIMD (64 KB of LDS per block)."""
+    path = os.path.join(ROOT, "lbmcl_amd", "libmembw.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.membw_pattern.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)]
+    out = {"dim": 512, "stride": stride, "bytes_per_cell": 152}
+    for name, lds in (("full_occupancy", 0), ("two_waves_per_simd", 65536)):
+        ms = (ctypes.c_double * reps)()
+        if L.membw_pattern(0, 512, stride, lds, reps, ms) != 0:
+            return None
+        best = min(ms[1:])
+        out[name] = {"best_ms": best, "gbs": 510 ** 3 * 152 / (best / 1e3) / 1e9}
+    return out
+
+
 def cpu_baseline(dim, steps, threads, stride):
     """The CPU oracle (reference push form, OpenMP) on `steps` iterations of the same lattice."""
     import oracle as O
@@ -240,6 +260,7 @@ def main():
 
     log("[bench] copy bandwidth")
     copy = copy_bandwidth()
+    pattern = access_pattern_bandwidth(stride) if dim == 512 else None
     cpu = None
     if args.cpu_steps > 0 and rank == 0:
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle iterations at {dim}^3")
@@ -277,6 +298,7 @@ def main():
         "variants": variants,
         "adjoint": adjoint,
         "copy_bandwidth": copy,
+        "access_pattern_bandwidth": pattern,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -69,6 +69,13 @@ extern "C" {
  * 8 in FP64, 32 with 16-bit storage) and dim >= max(that extent, 8).
  * LBM_KERNEL_AUTO selects it when stride * stored bytes <= 32. */
 #define LBM_KERNEL_TILE 5
+/* the row-pipeline kernel: a persistent grid walking whole rows, each row's
+ * 19 source spans staged in LDS by direct global-to-LDS loads issued one row
+ * ahead; needs dim 128, 256, 512 or 1024 with two rows of every population
+ * within 160 KB of LDS and a wavefront per row (FP32 and FP64 from 128, 16-bit
+ * from 256, all up to 512, FP32 and 16-bit at 1024) and
+ * stride >= 16 / bytes per stored value. */
+#define LBM_KERNEL_LDS 6
 
 /* lbm_config.optimize: arithmetic form of the collision (see lbm_compute).
  * The zero value is the reference's own arithmetic: with native storage,

@@ -24,6 +24,7 @@ LBM_ERR_STATE = -5
 LBM_F32, LBM_F64 = 0, 1
 LBM_STORE_NATIVE, LBM_STORE_FP16_SHIFT, LBM_STORE_BF16_SHIFT = 0, 1, 2
 LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_VEC8, LBM_KERNEL_TILE = 0, 1, 2, 3, 4, 5
+LBM_KERNEL_LDS = 6
 # lbm_config.optimize: arithmetic form of the collision.  The zero value is the
 # reference's own arithmetic (kernels.cl as ROCm's OpenCL compiler evaluates it;
 # 16-bit storage always computes in the fused form); EXACT is the CPU oracle's.

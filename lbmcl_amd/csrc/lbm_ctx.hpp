@@ -253,7 +253,22 @@ inline int dev_alloc(void** p, size_t bytes, const char* what) {
     return LBM_OK;
 }
 
-enum KernelKind { K_GENERIC, K_ROW, K_VEC, K_VEC8, K_TILE };
+enum KernelKind { K_GENERIC, K_ROW, K_VEC, K_VEC8, K_TILE, K_LDS };
+
+// The row-pipeline kernel (lbm_collide_stream_lds): whole rows of 128 to 1024
+// cells staged in two LDS buffers, which must fit one CU's 160 KB; its direct
+// 16-byte global-to-LDS loads need stride >= 16 / bytes per stored value.
+inline int64_t lds_buffer_bytes(int64_t n, int64_t ssize) { return 2 * 19 * (n + 2 * (16 / ssize)) * ssize; }
+inline bool lds_possible_for(int64_t n, int64_t stride, int64_t ssize) {
+    return (n == 128 || n == 256 || n == 512 || n == 1024) && n * ssize >= 512 &&  // a wavefront per row at least
+           stride >= 16 / ssize && lds_buffer_bytes(n, ssize) <= 160 * 1024;
+}
+// its persistent grid: the blocks (at most 8) that fit one CU's LDS, times the
+// MI355X's 256 CUs
+inline int64_t lds_grid(int64_t n, int64_t ssize) {
+    const int64_t per_cu = std::min<int64_t>(8, (160 * 1024) / (lds_buffer_bytes(n, ssize) + 256));
+    return 256 * std::max<int64_t>(1, per_cu);
+}
 
 // The tile kernel's shape for the stored population size.
 template <class S> inline void tile_dims(int64_t* d) {
@@ -302,6 +317,7 @@ inline KernelKind kernel_kind(const StepPlan* c) {
         case LBM_KERNEL_VEC: return K_VEC;
         case LBM_KERNEL_VEC8: return K_VEC8;
         case LBM_KERNEL_TILE: return K_TILE;
+        case LBM_KERNEL_LDS: return K_LDS;
         default: {
             if (tile_preferred(c)) return K_TILE;
             const KernelKind p = preferred_vec(c);
@@ -339,6 +355,7 @@ inline int generic_block(const StepPlan* c) {
 inline int64_t compute_blocks(const StepPlan* c) {
     const KernelKind k = kernel_kind(c);
     if (k == K_TILE) return (c->n / tile_b(c)) * (c->n / tile_ty(c)) * (c->n / tile_tz(c));
+    if (k == K_LDS) return lds_grid(c->n, int64_t(c->ssize));
     if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n);  // all rows
     if (k == K_ROW) return row_gx(c) * int64_t(c->n - 2) * int64_t(c->n - 2);         // interior rows
     const int64_t b = generic_block(c);
@@ -379,8 +396,15 @@ inline int plan_from_config(const lbm_config& k, StepPlan* p) {
     if (k.optimize == LBM_ARITH_EXACT && k.storage != LBM_STORE_NATIVE)
         return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the fused form)");
     if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
-        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE)
+        k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE &&
+        k.kernel != LBM_KERNEL_LDS)
         return fail(LBM_ERR_INVALID, "unknown kernel selection");
+    if (k.kernel == LBM_KERNEL_LDS) {
+        const int64_t ss = (k.storage != LBM_STORE_NATIVE) ? 2 : (k.precision == LBM_F64 ? 8 : 4);
+        if (!lds_possible_for(k.dim, k.stride, ss))
+            return fail(LBM_ERR_INVALID, "the row-pipeline kernel needs dim 128, 256, 512 or 1024 (two rows of every "
+                                         "population within 160 KB of LDS) and stride >= 16 / bytes per stored value");
+    }
     if (k.kernel == LBM_KERNEL_TILE) {
         int64_t d[3];
         if (k.storage != LBM_STORE_NATIVE) tile_dims<uint16_t>(d);
@@ -486,7 +510,23 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
         }
     }
     const KernelKind kk = kernel_kind(c);
-    if (kk == K_TILE) {
+    if (kk == K_LDS) {
+        const dim3 grid(unsigned(lds_grid(c->n, int64_t(c->ssize))));
+        using S = typename lbm::Codec<T, KIND>::S;
+        if (c->n == 128) {
+            if constexpr (lbm::LdsRow<S, 128>::WAVES >= 1)
+                launch_on(c, lbm::lbm_collide_stream_lds<T, KIND, ARITH, 128>, grid,
+                          dim3(unsigned(lbm::LdsRow<S, 128>::THREADS)), e0, e1, a);
+        } else if (c->n == 256)
+            launch_on(c, lbm::lbm_collide_stream_lds<T, KIND, ARITH, 256>, grid,
+                      dim3(unsigned(lbm::LdsRow<S, 256>::THREADS)), e0, e1, a);
+        else if (c->n == 512)
+            launch_on(c, lbm::lbm_collide_stream_lds<T, KIND, ARITH, 512>, grid,
+                      dim3(unsigned(lbm::LdsRow<S, 512>::THREADS)), e0, e1, a);
+        else if constexpr (2 * 19 * (1024 + 2 * (16 / sizeof(S))) * sizeof(S) <= 160 * 1024)
+            launch_on(c, lbm::lbm_collide_stream_lds<T, KIND, ARITH, 1024>, grid,
+                      dim3(unsigned(lbm::LdsRow<S, 1024>::THREADS)), e0, e1, a);
+    } else if (kk == K_TILE) {
         const dim3 grid(unsigned(c->n / tile_b(c)), unsigned(c->n / tile_ty(c)), unsigned(c->n / tile_tz(c)));
         const dim3 block(unsigned(tile_b(c) * tile_ty(c) * tile_tz(c)));
         launch_on(c, lbm::lbm_collide_stream_tile<T, KIND, ARITH>, grid, block, e0, e1, a);

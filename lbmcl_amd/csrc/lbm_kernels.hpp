@@ -26,6 +26,17 @@
 
 #pragma clang fp contract(off)
 
+// Loads and stores of the lattices in the vector and row-pipeline kernels:
+// streaming (nontemporal) by default -- every slot is read and written once
+// per step.  A build can define LBM_LATTICE_CACHED to compare cached accesses.
+#ifndef LBM_LATTICE_CACHED
+#define LBM_LAT_LOAD(p) __builtin_nontemporal_load(p)
+#define LBM_LAT_STORE(v, p) __builtin_nontemporal_store(v, p)
+#else
+#define LBM_LAT_LOAD(p) (*(p))
+#define LBM_LAT_STORE(v, p) (*(p) = (v))
+#endif
+
 namespace lbm {
 
 // ---------------------------------------------------------------------------
@@ -695,7 +706,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     // are streaming (nontemporal: 1.5-2.5% faster than cached loads at 512^3)
     SV A[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) A[i] = __builtin_nontemporal_load(reinterpret_cast<const SV*>(src_row(i) + off0));
+    for (int i = 0; i < Q; ++i) A[i] = LBM_LAT_LOAD(reinterpret_cast<const SV*>(src_row(i) + off0));
     // wavefront-end elements: from the neighbouring wavefront of the block
     // through LDS, from memory only at the block's ends
     __shared__ S xe_up[16][5];  // per wavefront: lane 63's last element, e_x = +1 populations
@@ -798,7 +809,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         SV o;
 #pragma unroll
         for (int j = 0; j < V; ++j) o[j] = g[i][j];
-        __builtin_nontemporal_store(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+        LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
     }
 
     if (a.update_macro) {
@@ -835,6 +846,192 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         if (threadIdx.x == 0) {
             a.partials[2 * lb] = m;
             a.partials[2 * lb + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row-pipeline kernel: a persistent grid whose blocks walk interior rows of
+// NX = N cells, staging each row's 19 source spans in LDS with gfx950's
+// direct global-to-LDS loads (global_load_lds_dwordx4).  Those loads hold no
+// VGPRs while in flight, so a block issues the next row's loads before it
+// computes the current one: the memory system always has a row per block in
+// flight, at two or four waves per SIMD, where the vector kernel keeps its
+// loads in registers and needs four waves per SIMD to hide its compute.
+//
+// LDS: two buffers of 19 spans; span i holds the source row of population i,
+// cells 0 .. NX-1, between 16-byte pads, so a cell reads its population i at
+// x - e_x without bounds tests (x = 0 and x = N-1 are wall cells whose
+// gathered values are not used: they store their initial values, wall_p0).
+// One load instruction moves 1 KB: 64 lanes x 16 bytes of one span.
+//
+// Per row k of the block: wait for the loads of row k (issued during row
+// k-1; the stores of row k-1 may still be in flight), barrier, copy the row's
+// gathers from LDS into registers, issue the loads of row k+1 into the other
+// buffer (everyone read it before this barrier), compute, store.  Wall rows
+// are not visited; block b takes interior rows b, b + grid, b + 2 grid, ...
+// (any grid size is correct: blocks do not depend on each other).  Each
+// block's diagnostics cover all its rows.
+// Requirements: N = NX, stride >= 16 / sizeof(stored) (whole 16-byte pieces
+// are contiguous).
+// ---------------------------------------------------------------------------
+// One direct global-to-LDS load of 16 bytes per lane (gfx950
+// global_load_lds_dwordx4): lane l's bytes land at LDS byte address
+// lds_base + 16 l (lds_base wavefront-uniform, through M0).  Written as inline
+// assembly so that the compiler, which does not see the LDS write, inserts no
+// wait of its own for it: the kernel waits with s_waitcnt vmcnt before the
+// barrier that precedes the LDS reads.  The instruction writes no VGPR.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"  // m0 is reserved: the clobber documents the write
+__device__ __forceinline__ void dma16(const void* src, uint32_t lds_base) {
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(src), "s"(lds_base)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+template <class S> __device__ __forceinline__ uint32_t lds_address(S* p) {
+    return uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) S*)(p)));
+}
+
+template <class S, int NX> struct LdsRow {
+    static constexpr int PAD = 16 / int(sizeof(S));             // elements of a 16-byte pad
+    static constexpr int PITCH = NX + 2 * PAD;                    // elements per span
+    static constexpr int V = 8 / int(sizeof(S));                  // cells per lane (8-byte lanes)
+    static constexpr int THREADS = NX / V;
+    static constexpr int SPAN_LANES = NX * int(sizeof(S)) / 16;   // 16-byte pieces per span
+    static constexpr int LANES = SPAN_LANES < 64 ? SPAN_LANES : 64;  // active lanes per load instruction
+    static constexpr int CPI = LANES * 16 / int(sizeof(S));       // cells per load instruction
+    static constexpr int CHUNKS = NX / CPI;                       // instructions per span
+    static constexpr int LOADS = Q * CHUNKS;                      // instructions per row
+    static constexpr int WAVES = THREADS / 64;
+    static constexpr int BUF = Q * PITCH;                         // elements per buffer
+};
+
+template <class T, int KIND, int ARITH, int NX>
+__global__ __launch_bounds__((LdsRow<typename Codec<T, KIND>::S, NX>::THREADS)) void lbm_collide_stream_lds(StepArgs<T> a) {
+    using C = Codec<T, KIND>;
+    using S = typename C::S;
+    using L = LdsRow<S, NX>;
+    constexpr int V = L::V;
+    typedef S SV __attribute__((ext_vector_type(V)));
+    __shared__ __align__(16) S lds[2 * L::BUF];
+    const C codec{a.scale, a.inv_scale};
+    constexpr int32_t n = NX;
+    const int64_t smask = a.stride - 1;
+    const uint32_t smask32 = uint32_t(smask);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const int32_t x0 = int32_t(tid) * V;
+    auto voff = [&](int32_t xx) -> uint32_t {  // slot(N r + x) - slot(N r)
+        const uint32_t u = uint32_t(xx);
+        return 19u * u - 18u * (u & smask32);
+    };
+    const uint32_t off0 = voff(x0);
+    const int64_t rows = int64_t(n - 2) * (n - 2);
+    const int64_t first = blockIdx.x, step = gridDim.x;
+
+    // loads of row j (interior index) into buffer b: wave w issues instructions m = w, w + WAVES, ...
+    const int wave_u = __builtin_amdgcn_readfirstlane(int(wave));
+    auto issue = [&](int64_t j, int b) {
+        const int64_t r = (1 + j % (n - 2)) + int64_t(n) * (1 + j / (n - 2));  // y + N z
+#pragma unroll
+        for (int m = 0; m < L::LOADS; ++m) {
+            if (m % L::WAVES == wave_u) {
+                const int i = m / L::CHUNKS, h = m % L::CHUNKS;
+                // source row of population i (wavefront-uniform)
+                const int64_t c0 = int64_t(n) * (r - ey_of(i) - int64_t(n) * ez_of(i));
+                const S* row = static_cast<const S*>(a.src) + (19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride);
+                const int32_t xl = h * L::CPI + int32_t(lane) * (16 / int(sizeof(S)));
+                S* ldst = lds + b * L::BUF + i * L::PITCH + L::PAD + h * L::CPI;
+                if (L::LANES == 64 || int(lane) < L::LANES) dma16(row + voff(xl), lds_address(ldst));
+            }
+        }
+    };
+
+    double m_acc = 0.0, e_acc = 0.0;
+    int k = 0;
+    if (first < rows) issue(first, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the first row's loads
+    for (int64_t j = first; j < rows; j += step, ++k) {
+        const int b = k & 1;
+        // Vector memory operations complete in issue order, and each wave
+        // issued this row's loads before the previous row's 19 stores: once at
+        // most 19 operations are outstanding, the loads have landed.  The
+        // barrier then makes every wave's part visible.  (Inline assembly: one
+        // s_barrier on every path, and no compiler-inserted wait for the stores.)
+        asm volatile("s_waitcnt vmcnt(19)\n\ts_barrier" ::: "memory");
+        const int32_t y = int32_t(1 + j % (n - 2)), z = int32_t(1 + j / (n - 2));
+        const int64_t r = int64_t(y) + int64_t(n) * z;
+        // gathers: g[i][c] is population i arriving at cell x0 + c, read at x0 + c - e_x
+        S g[Q][V];
+        const S* lb = lds + b * L::BUF + L::PAD;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const S* sp = lb + i * L::PITCH + x0 - ex_of(i);
+            if (ex_of(i) == 0) {
+                const SV v = *reinterpret_cast<const SV*>(sp);
+#pragma unroll
+                for (int c = 0; c < V; ++c) g[i][c] = v[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < V; ++c) g[i][c] = sp[c];
+            }
+        }
+        if (j + step < rows) issue(j + step, b ^ 1);
+
+        const int32_t rf = row_flags(y, z, n);
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            const int32_t t = finish_type(x_flags(x0 + c, n) | rf);
+            T f[Q];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] = value_in<ARITH>(codec, g[i][c], w_of<T>(i));
+            T rho = T(0), ux = T(0), uy = T(0), uz = T(0);
+            collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
+            if (a.force) {
+                const int64_t cc = int64_t(x0 + c) + int64_t(n) * r;
+                add_force<T>(t, f, a.force[cc], a.force[a.n3 + cc], a.force[2 * a.n3 + cc]);
+            }
+            const bool store = is_store_macro(t);
+            if (a.diagnostics && store) {
+                double dm, de;
+                diag_terms<T>(rho, ux, uy, uz, dm, de);
+                m_acc += dm;
+                e_acc += de;
+            }
+            if (a.update_macro) {
+                const int64_t cc = int64_t(x0 + c) + int64_t(n) * r;
+                const T qnan = T(__builtin_nan(""));
+                a.rho[cc] = store ? rho : qnan;
+                a.u[cc] = store ? ux : qnan;
+                a.u[a.n3 + cc] = store ? uy : qnan;
+                a.u[2 * a.n3 + cc] = store ? uz : qnan;
+            }
+            const bool wl = (x0 + c == 0), wr = (x0 + c == n - 1);  // the row's wall cells keep their initial values
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if (wl) f[i] = wall_p0<T, ARITH>(i, -1, y, z, n, a.lid_eq);
+                if (wr) f[i] = wall_p0<T, ARITH>(i, +1, y, z, n, a.lid_eq);
+            }
+#pragma unroll
+            for (int i = 0; i < Q; ++i) g[i][c] = value_out<ARITH>(codec, f[i], w_of<T>(i));
+        }
+        const int64_t oc0 = int64_t(n) * r;
+        S* orow = static_cast<S*>(a.dst) + (19 * oc0 - 18 * (oc0 & smask));
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            SV o;
+#pragma unroll
+            for (int c = 0; c < V; ++c) o[c] = g[i][c];
+            LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+        }
+    }
+    if (a.diagnostics) {
+        __shared__ double red[2 * 16];
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        block_reduce2(m_acc, e_acc, red);
+        if (threadIdx.x == 0) {
+            a.partials[2 * blockIdx.x] = m_acc;
+            a.partials[2 * blockIdx.x + 1] = e_acc;
         }
     }
 }

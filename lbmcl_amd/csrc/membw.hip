@@ -1,4 +1,5 @@
-// membw.hip -- a plain device-to-device copy kernel of known size.
+// membw.hip -- calibration kernels: a plain device-to-device copy of known
+// size, and the data movement of one D3Q19 step without its arithmetic.
 //
 // Used by bench.py for two things: the sustained copy bandwidth of the GPU it
 // runs on (read + write bytes / kernel time, HIP events on the kernel's own
@@ -29,9 +30,84 @@ __global__ __launch_bounds__(256) void copy_kernel(const U4* __restrict__ src, U
     if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// The D3Q19 pull step's data movement without its arithmetic: one block per
+// interior row of an N^3 lattice in the CSoA layout with stride S, each lane
+// two consecutive cells (8-byte lanes); 19 aligned loads from the source rows
+// (y - e_y, z - e_z) of each population, then 19 aligned stores to the row's
+// own slots -- the access pattern of the collide-stream kernels at full
+// occupancy or, with `lds_bytes` of dynamic LDS per block, at fewer waves.
+// The values are moved, not combined (no e_x shift: the shifted element moves
+// within the same cache lines).
+__global__ __launch_bounds__(256) void pattern_kernel(const float* __restrict__ src, float* __restrict__ dst, int32_t n,
+                                                      int64_t stride) {
+    extern __shared__ float pad[];
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    const uint32_t nrow = uint32_t(n - 2) * uint32_t(n - 2);
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+    if (b >= nrow) return;
+    const int64_t y = 1 + b % uint32_t(n - 2), z = 1 + b / uint32_t(n - 2);
+    const int64_t r = y + int64_t(n) * z;
+    const int64_t smask = stride - 1;
+    const uint32_t x0 = threadIdx.x * 2u;
+    const uint32_t off = 19u * x0 - 18u * (x0 & uint32_t(smask));
+    const int ey[19] = {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 1, 0, -1, 0, 1, 0, -1};
+    const int ez[19] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, -1, -1, -1, 1, 1, 1, 1};
+    F2 v[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int64_t c0 = int64_t(n) * (r - ey[i] - int64_t(n) * ez[i]);
+        v[i] = __builtin_nontemporal_load(reinterpret_cast<const F2*>(src + 19 * c0 - 18 * (c0 & smask) + i * stride + off));
+    }
+    if (threadIdx.x == 0xffffffffu) pad[0] = v[0][0];  // keeps the dynamic LDS (occupancy control) allocated
+    const int64_t oc0 = int64_t(n) * r;
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+        __builtin_nontemporal_store(v[i], reinterpret_cast<F2*>(dst + 19 * oc0 - 18 * (oc0 & smask) + i * stride + off));
+}
+
 }  // namespace
 
 extern "C" {
+
+// The access pattern of one FP32 collide-stream step on an n^3 lattice (n = 512:
+// one 256-thread block per row) with CSoA stride `stride`, `reps` times between
+// two fresh lattices; ms[r] holds the duration of repetition r.  lds_bytes of
+// dynamic LDS per block limit the waves per SIMD (0: none).
+int membw_pattern(int device, int n, int64_t stride, int lds_bytes, int reps, double* ms) {
+    if (n != 512 || stride < 2 || (stride & (stride - 1)) || reps <= 0 || !ms || lds_bytes < 0 || lds_bytes > 65536)
+        return -1;
+    if (hipSetDevice(device) != hipSuccess) return -2;
+    const size_t bytes = size_t(19) * size_t(n) * size_t(n) * size_t(n) * sizeof(float);
+    const unsigned rows = unsigned(n - 2) * unsigned(n - 2);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = -3;
+    }
+    if (!rc && (hipMemsetAsync(a, 0, bytes, s) != hipSuccess || hipMemsetAsync(b, 0, bytes, s) != hipSuccess)) rc = -4;
+    for (int r = 0; !rc && r < reps; ++r) {
+        if (hipEventRecord(e0, s) != hipSuccess) rc = -4;
+        hipLaunchKernelGGL(pattern_kernel, dim3(rows), dim3(256), size_t(lds_bytes), s,
+                           static_cast<const float*>((r & 1) ? b : a), static_cast<float*>((r & 1) ? a : b), n, stride);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess)
+            rc = -4;
+        float t = 0.f;
+        if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = -4;
+        ms[r] = double(t);
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
 
 // Copy `bytes` (a multiple of 16, at most 2^31 * 256 * 16) `reps` times between
 // two fresh device buffers; on success ms[r] holds the duration of repetition r.

@@ -72,6 +72,13 @@ CASES = [
     (128, 128, "f32", capi.LBM_KERNEL_VEC8, 6),
     (64, 64 ** 3, "f64", capi.LBM_KERNEL_VEC8, 8),
     (64, 1, "f64", capi.LBM_KERNEL_VEC8, 8),
+    # the row-pipeline kernel (rows staged in LDS by direct loads, one row ahead)
+    (256, 4096, "f32", capi.LBM_KERNEL_LDS, 5),
+    (256, 4, "f32", capi.LBM_KERNEL_LDS, 5),
+    (512, 4096, "f32", capi.LBM_KERNEL_LDS, 2),
+    (256, 2, "f64", capi.LBM_KERNEL_LDS, 4),
+    (256, 256 ** 3, "f64", capi.LBM_KERNEL_LDS, 3),
+    (512, 512, "f64", capi.LBM_KERNEL_LDS, 2),
     # the tile kernel (small strides, LDS-staged spans)
     (32, 1, "f32", capi.LBM_KERNEL_TILE, 12),
     (32, 16, "f32", capi.LBM_KERNEL_TILE, 12),
@@ -133,7 +140,8 @@ def test_fused_collision_matches_oracle(n, stride, precision, kernel, steps):
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
 @pytest.mark.parametrize("n,stride,kernel,steps", [(16, 16, capi.LBM_KERNEL_GENERIC, 12), (64, 64, capi.LBM_KERNEL_ROW, 12),
                                                    (256, 256, capi.LBM_KERNEL_VEC8, 4), (512, 512, capi.LBM_KERNEL_VEC, 2),
-                                                   (64, 1, capi.LBM_KERNEL_TILE, 8), (128, 32, capi.LBM_KERNEL_TILE, 4)])
+                                                   (64, 1, capi.LBM_KERNEL_TILE, 8), (128, 32, capi.LBM_KERNEL_TILE, 4),
+                                                   (256, 8, capi.LBM_KERNEL_LDS, 4), (512, 4096, capi.LBM_KERNEL_LDS, 2)])
 def test_shifted_storage_matches_oracle(storage, n, stride, kernel, steps):
     label = f"{storage} n={n} s={stride} k={kernel}"
     g, o = _pair(n, stride, "f32", storage=storage, kernel=kernel)
@@ -155,7 +163,9 @@ def test_shifted_storage_matches_oracle(storage, n, stride, kernel, steps):
                                                        (256, 256, "f32", capi.LBM_KERNEL_VEC),
                                                        (128, 128, "f64", capi.LBM_KERNEL_VEC),
                                                        (64, 1, "f32", capi.LBM_KERNEL_TILE),
-                                                       (32, 2, "f64", capi.LBM_KERNEL_TILE)])
+                                                       (32, 2, "f64", capi.LBM_KERNEL_TILE),
+                                                       (256, 4096, "f32", capi.LBM_KERNEL_LDS),
+                                                       (256, 64, "f64", capi.LBM_KERNEL_LDS)])
 def test_diagnostics(n, stride, precision, kernel):
     g, o = _pair(n, stride, precision, kernel=kernel)
     with g:
@@ -284,13 +294,14 @@ def test_full_size_512_f32():
 def test_full_size_1024_f32_vector_matches_generic():
     """1024^3 FP32 -- 163 GB of lattices, element offsets beyond 2^32 (19 N^3 =
     2.0e10), a size the reference's int index cannot address and the CPU oracle
-    is too large for: the 8-byte-lane vector kernel (two blocks per row) and the
-    one-thread-per-cell generic kernel, both bit-identical to the oracle at
+    is too large for: the 8-byte-lane vector kernel (two blocks per row), the
+    row-pipeline kernel (whole 1024-cell rows in LDS) and the
+    one-thread-per-cell generic kernel, all bit-identical to the oracle at
     smaller sizes, agree bit for bit on rho and u after two steps, and on the
     FP64 mass and energy to their summation order."""
     n = 1024
     out = {}
-    for kernel in (capi.LBM_KERNEL_VEC8, capi.LBM_KERNEL_GENERIC):
+    for kernel in (capi.LBM_KERNEL_VEC8, capi.LBM_KERNEL_GENERIC, capi.LBM_KERNEL_LDS):
         print(f"1024^3 FP32 stride 4096 kernel {kernel}", flush=True)
         with capi.Context(n, stride=4096, precision="f32", kernel=kernel) as g:
             g.initialize()
@@ -298,6 +309,11 @@ def test_full_size_1024_f32_vector_matches_generic():
             g.compute(2, update_macro=True, diagnostics=True)
             out[kernel] = (g.read_rho(), g.read_u(), g.read_diagnostics())
     (rv, uv, dv), (rg, ug, dg) = out[capi.LBM_KERNEL_VEC8], out[capi.LBM_KERNEL_GENERIC]
+    rl, ul, dl = out.pop(capi.LBM_KERNEL_LDS)
+    assert nan_equal(rl, rg), mismatch_count(rl, rg)
+    assert nan_equal(ul, ug), mismatch_count(ul, ug)
+    assert abs(dl[0] - dg[0]) <= 1e-12 * abs(dg[0]) and abs(dl[1] - dg[1]) <= 1e-10 * abs(dg[1])
+    del rl, ul
     assert nan_equal(rv, rg), mismatch_count(rv, rg)
     assert int(np.isfinite(rv).sum()) > (n - 4) ** 3
     del rv, rg

@@ -129,7 +129,8 @@ def test_opencl_arithmetic_other_layouts(tmp_path, precision, stride):
 
 @pytest.mark.parametrize("precision,n,stride,kernel", [
     ("f32", 64, 64, capi.LBM_KERNEL_GENERIC), ("f32", 64, 64, capi.LBM_KERNEL_ROW),
-    ("f32", 256, 64, capi.LBM_KERNEL_VEC), ("f32", 64, 1, capi.LBM_KERNEL_TILE),
+    ("f32", 256, 64, capi.LBM_KERNEL_VEC), ("f32", 64, 1, capi.LBM_KERNEL_TILE), ("f32", 256, 64, capi.LBM_KERNEL_LDS),
+    ("f64", 256, 64, capi.LBM_KERNEL_LDS),
     ("f64", 64, 64, capi.LBM_KERNEL_GENERIC), ("f64", 64, 64, capi.LBM_KERNEL_ROW),
     ("f64", 64, 1, capi.LBM_KERNEL_TILE),
 ])
