@@ -297,6 +297,29 @@ int lbm_set_force(lbm_ctx* ctx, const void* force);
  * in the reference. */
 int lbm_adjoint(lbm_ctx* ctx, const lbm_adjoint_config* config, lbm_adjoint_result* result, double* grad_force);
 
+/* One collide-stream iteration that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs the kernels of lbm_compute(ctx, it, update_macro, 1) -- same kernel
+ * selection (cfg->kernel), arithmetic (cfg->optimize) and results, bit for
+ * bit -- on caller-owned device memory and enqueues on `stream` (a
+ * hipStream_t; NULL: the null stream) without waiting:
+ *   src, dst  the lattice read and the lattice written, pull form as stored
+ *             (19 * dim^3 values of the storage type, CSoA with cfg->stride;
+ *             lbm_read_state/lbm_write_state move the same bytes), distinct;
+ *   rho, u    dim^3 and 3 * dim^3 values of the arithmetic type, written on
+ *             FLUID/MOVING cells when update_macro != 0 (may be NULL otherwise);
+ *   diag      2 doubles of device memory: mass and kinetic energy (FP64).
+ * cfg is validated like lbm_create's (struct_size, dim, stride, ...) and
+ * cfg->device selects the device (the caller's current device is restored).
+ * The per-block partial sums live in a fixed array of the library on each
+ * device, 2^19 blocks (8 MiB, enough for every form at dim 512); a
+ * configuration whose launch has more blocks is rejected with
+ * LBM_ERR_INVALID.  Calls that overlap on one device share that array and
+ * race: serialise them (one stream, or events).  No body force. */
+int lbm_compute_noalloc(const lbm_config* cfg, const void* src, void* dst, void* rho, void* u, int32_t update_macro,
+                        double* diag, void* stream);
+
 /* Text of the calling thread's last error ("" if none). */
 const char* lbm_last_error(void);
 

@@ -54,7 +54,7 @@ def build_lib():
     _run([HIPCC, *HIP_FLAGS, "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-c", "-I", INCLUDE, "-I", CSRC,
           os.path.join(CSRC, "lbm_opencl_arith.hip"), "-o", opencl_obj])
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
-          os.path.join(CSRC, "lbm_capi.hip"), os.path.join(CSRC, "lbm_adjoint.hip"),
+          os.path.join(CSRC, "lbm_capi.hip"), os.path.join(CSRC, "lbm_adjoint.hip"), os.path.join(CSRC, "lbm_noalloc.hip"),
           "-x", "none", opencl_obj, "-o", LIB])
 
 

@@ -44,6 +44,7 @@ EXPORTS = (
     "lbm_read_f", "lbm_read_state", "lbm_write_state", "lbm_read_diagnostics", "lbm_get_constants",
     "lbm_event_count", "lbm_event_get", "lbm_total_time_ms", "lbm_kernels_time_ms", "lbm_clear_events",
     "lbm_set_force", "lbm_adjoint", "lbm_snapshot", "lbm_snapshot_wait", "lbm_snapshot_release", "lbm_last_error",
+    "lbm_compute_noalloc",
 )
 
 LBM_SNAPSHOT_MACRO, LBM_SNAPSHOT_F, LBM_SNAPSHOT_MAP = 0, 1, 2
@@ -146,6 +147,7 @@ def _declare(L):
         "lbm_snapshot": [vp, i32, i32, P(i64)],
         "lbm_snapshot_wait": [vp, i64, P(vp), P(i64)],
         "lbm_snapshot_release": [vp, i64],
+        "lbm_compute_noalloc": [P(LbmConfig), vp, vp, vp, vp, i32, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
