@@ -117,6 +117,7 @@ def _declare(L):
     L.lbmo_pull_compute_f64.argtypes = [P, vp, vp, vp, vp, i32, vp, vp]
     L.lbmo_pull_view_f32.argtypes = [P, i32, vp, vp]
     L.lbmo_pull_view_f64.argtypes = [P, vp, vp]
+    L.lbmo_collide_pair.argtypes = [P, i32, vp, vp, vp, vp, vp]
     for name in ("lbmo_f32_to_f16", "lbmo_f16_to_f32", "lbmo_f32_to_bf16", "lbmo_bf16_to_f32"):
         getattr(L, name).argtypes = [vp, vp, i64]
 

@@ -825,6 +825,28 @@ void lbmo_diagnostics_f64(const lbmo_params* p, const double* rho, const double*
     diagnostics<double>(*p, rho, u, out);
 }
 
+// One cell's local step in two forms, for pinning the deviation form against
+// the full-value form: f (19 populations) through collide in FP64 (exact
+// expressions) into out_full, and d = f - w_i through collide_delta (FP32) into
+// out_delta; force may be null.  out_macro: rho, ux, uy, uz of each form.
+void lbmo_collide_pair(const lbmo_params* p, int32_t t, const double* f, const float* force, double* out_full,
+                       float* out_delta, double* out_macro) {
+    Params q = *p;
+    q.fast = 0;
+    const Consts<double> kd(q);
+    const Consts<float> kf(q);
+    double g[3] = {0, 0, 0};
+    if (force) for (int j = 0; j < 3; ++j) g[j] = force[j];
+    for (int i = 0; i < Q; ++i) out_full[i] = f[i];
+    double r, ux, uy, uz;
+    collide(kd, t, out_full, &r, &ux, &uy, &uz, force ? g : nullptr);
+    for (int i = 0; i < Q; ++i) out_delta[i] = float(f[i] - double(kf.w[i]));
+    float rf, xf, yf, zf;
+    collide_delta(kf, t, out_delta, &rf, &xf, &yf, &zf, force);
+    const double m[8] = {r, ux, uy, uz, rf, xf, yf, zf};
+    for (int j = 0; j < 8; ++j) out_macro[j] = m[j];
+}
+
 // Element-wise codec helpers (for pinning the conversions against numpy/torch).
 void lbmo_f32_to_f16(const float* in, uint16_t* out, int64_t n) {
     for (int64_t i = 0; i < n; ++i) out[i] = f32_to_f16_rne(in[i]);
