@@ -38,6 +38,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {("f32", "native"): 152, ("f64", "native"): 304, ("f32", "fp16"): 76, ("f32", "bf16"): 76}
+VARIANT_KEYS = ("mlups", "mlups_with_events", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
+                "bytes_per_update", "steps")
 
 
 def log(msg):
@@ -46,27 +48,33 @@ def log(msg):
 
 def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=None, kernel=0, xcd_remap=True,
                 arith=0):
-    """`arith`: lbm_config.optimize (0 the reference's OpenCL rounding, 1 fused, 2 exact)."""
+    """Two passes over the same iterations: with the reference's per-launch events
+    (the compute kernels' own durations: kernel time, roofline) and without them
+    (the wall time of the launches back to back: `mlups`, `ms_per_step`).
+    `arith`: lbm_config.optimize (0 the reference's OpenCL rounding, 1 fused, 2 exact)."""
     from lbmcl_amd import capi
     stride = stride or dim ** 3
-    with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=True,
-                      xcd_remap=xcd_remap, arith=arith) as g:
-        g.initialize()
-        it = 1
-        if warmup:
-            g.compute_range(it, warmup, 0)
-            it += warmup
-        g.finish()
-        g.clear_events()
-        t0 = time.perf_counter()
-        g.compute_range(it, steps, 0)
-        g.finish()
-        t1 = time.perf_counter()
-        kernel_ms = g.kernels_time_ms()
-        events = g.events()
-        total_ms = g.total_time_ms()
+    walls = {}
+    for profile in (True, False):
+        with capi.Context(dim, stride=stride, precision=precision, storage=storage, kernel=kernel, profile=profile,
+                          xcd_remap=xcd_remap, arith=arith) as g:
+            g.initialize()
+            it = 1
+            if warmup:
+                g.compute_range(it, warmup, 0)
+                it += warmup
+            g.finish()
+            g.clear_events()
+            t0 = time.perf_counter()
+            g.compute_range(it, steps, 0)
+            g.finish()
+            walls[profile] = time.perf_counter() - t0
+            if profile:
+                kernel_ms = g.kernels_time_ms()
+                events = g.events()
+                total_ms = g.total_time_ms()
     wet = (dim - 2) ** 3
-    wall_s = t1 - t0
+    wall_s = walls[False]
     avg_kernel_s = kernel_ms / 1e3 / steps
     b = BYTES_PER_UPDATE[(precision, storage)]
     achieved = wet * b / avg_kernel_s / 1e9
@@ -74,6 +82,7 @@ def run_lattice(dim, steps, warmup, precision="f32", storage="native", stride=No
         "dim": dim, "precision": precision, "storage": storage, "stride": stride, "steps": steps,
         "wall_s": wall_s, "ms_per_step": wall_s * 1e3 / steps,
         "mlups": wet * steps / wall_s / 1e6,
+        "mlups_with_events": wet * steps / walls[True] / 1e6,
         "kernel_mlups": wet * steps / (kernel_ms / 1e3) / 1e6,
         "avg_kernel_ms": kernel_ms / steps,
         "event_span_ms": total_ms,
@@ -225,8 +234,7 @@ def main():
             log(f"[bench] variant {name}")
             r = run_lattice(dim, args.variant_steps, max(5, args.warmup // 4), prec, st, stride, args.kernel,
                             not args.no_remap, arith)
-            variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
-                                                "bytes_per_update", "steps")}
+            variants[name] = {k: r[k] for k in VARIANT_KEYS}
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
         variants["fp16_speedup_vs_f32"] = variants["f32_fp16"]["mlups"] / main_run["mlups"]
         # the 128^3 configuration: MLUPS over iterations 100..1000 (99 untimed first)
@@ -234,8 +242,7 @@ def main():
             name = f"cavity_128_{prec}_it100_1000"
             log(f"[bench] variant {name}")
             r = run_lattice(128, 901, 99, prec, "native", 64, args.kernel, not args.no_remap)
-            variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs", "frac",
-                                                "bytes_per_update", "steps")}
+            variants[name] = {k: r[k] for k in VARIANT_KEYS}
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
         # 1024^3 FP32: 179 GB of HBM, a lattice the reference cannot index
         if dim == 512:
@@ -244,8 +251,7 @@ def main():
             log(f"[bench] variant {name}")
             try:
                 r = run_lattice(1024, 50, 5, "f32", "native", stride, args.kernel, not args.no_remap)
-                variants[name] = {k: r[k] for k in ("mlups", "kernel_mlups", "avg_kernel_ms", "achieved_gbs",
-                                                    "frac", "bytes_per_update", "steps")}
+                variants[name] = {k: r[k] for k in VARIANT_KEYS}
                 log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
             except capi.LbmError as e:  # e.g. less free HBM than the 179 GB it needs
                 variants[name] = {"skipped": str(e)}

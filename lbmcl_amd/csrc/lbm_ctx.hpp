@@ -343,6 +343,17 @@ inline int64_t row_block(const StepPlan* c) {
     return per_row < 256 ? per_row : 256;
 }
 inline int64_t row_gx(const StepPlan* c) { return (c->n / vec_width(c)) / row_block(c); }
+// Rows per block of the vector kernels: rows shorter than 256 threads share a
+// block (at most LBM_VEC_ROWS_PER_BLOCK of them), so short-row lattices launch
+// fewer, larger workgroups.
+#ifndef LBM_VEC_ROWS_PER_BLOCK
+#define LBM_VEC_ROWS_PER_BLOCK 4
+#endif
+inline int64_t vec_rows_per_block(const StepPlan* c) {
+    int64_t r = 256 / row_block(c);
+    if (r > LBM_VEC_ROWS_PER_BLOCK) r = LBM_VEC_ROWS_PER_BLOCK;
+    return r < 1 ? 1 : r;
+}
 
 inline int generic_block(const StepPlan* c) {
     int64_t b = 1;
@@ -356,7 +367,7 @@ inline int64_t compute_blocks(const StepPlan* c) {
     const KernelKind k = kernel_kind(c);
     if (k == K_TILE) return (c->n / tile_b(c)) * (c->n / tile_ty(c)) * (c->n / tile_tz(c));
     if (k == K_LDS) return lds_grid(c->n, int64_t(c->ssize));
-    if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n);  // all rows
+    if (k == K_VEC || k == K_VEC8) return row_gx(c) * int64_t(c->n) * int64_t(c->n) / vec_rows_per_block(c);  // all rows
     if (k == K_ROW) return row_gx(c) * int64_t(c->n - 2) * int64_t(c->n - 2);         // interior rows
     const int64_t b = generic_block(c);
     return (c->n3 + b - 1) / b;
@@ -535,12 +546,18 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
         a.log2gx = ilog2(a.gx);
         const int64_t nb = compute_blocks(c);
         const unsigned bx = unsigned(row_block(c));
-        if (kk == K_VEC)
-            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
-        else if (kk == K_VEC8)
-            launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
-        else
+        if (kk == K_VEC || kk == K_VEC8) {
+            const int64_t rpb = vec_rows_per_block(c);
+            a.log2tpr = ilog2(int64_t(bx));
+            a.log2rpb = ilog2(rpb);
+            const dim3 block(unsigned(int64_t(bx) * rpb));
+            if (kk == K_VEC)
+                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, ARITH>, dim3(unsigned(nb)), block, e0, e1, a);
+            else
+                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH>, dim3(unsigned(nb)), block, e0, e1, a);
+        } else {
             launch_on(c, lbm::lbm_collide_stream_row<T, KIND, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
+        }
     } else {
         const int b = generic_block(c);
         const int64_t nb = (c->n3 + b - 1) / b;

@@ -29,12 +29,15 @@
 // Loads and stores of the lattices in the vector and row-pipeline kernels:
 // streaming (nontemporal) by default -- every slot is read and written once
 // per step.  A build can define LBM_LATTICE_CACHED to compare cached accesses.
-#ifndef LBM_LATTICE_CACHED
-#define LBM_LAT_LOAD(p) __builtin_nontemporal_load(p)
-#define LBM_LAT_STORE(v, p) __builtin_nontemporal_store(v, p)
-#else
+#if defined(LBM_LATTICE_CACHED) || defined(LBM_LATTICE_CACHED_LOADS)
 #define LBM_LAT_LOAD(p) (*(p))
+#else
+#define LBM_LAT_LOAD(p) __builtin_nontemporal_load(p)
+#endif
+#if defined(LBM_LATTICE_CACHED) || defined(LBM_LATTICE_CACHED_STORES)
 #define LBM_LAT_STORE(v, p) (*(p) = (v))
+#else
+#define LBM_LAT_STORE(v, p) __builtin_nontemporal_store(v, p)
 #endif
 
 namespace lbm {
@@ -508,6 +511,8 @@ template <class T> struct StepArgs {
     int32_t xcd_remap;
     int32_t gx;           // row kernels: blocks per row
     int32_t log2gx;       // vector kernel: log2(gx)
+    int32_t log2tpr;      // vector kernel: log2(threads per row segment of a block)
+    int32_t log2rpb;      // vector kernel: log2(rows per block), > 0 for rows shorter than a block
     StepConsts<T> k;
     float scale, inv_scale;
     T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
@@ -654,8 +659,11 @@ __device__ __forceinline__ T wall_p0(int i, int side, int32_t y, int32_t z, int3
 
 // Occupancy target of the vector kernel: the waves per SIMD its register
 // budget is fitted to (512 VGPRs per SIMD lane / waves).
+#ifndef LBM_VEC8_NATIVE_WAVES
+#define LBM_VEC8_NATIVE_WAVES 4
+#endif
 template <class T, int KIND, int LB> struct VecWaves {
-    static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? 4 : 3);
+    static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? LBM_VEC8_NATIVE_WAVES : 3);
 };
 
 template <class T, int KIND, int LB, int ARITH>
@@ -669,21 +677,26 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     const int32_t n = a.n;
     // grid: gx blocks per row, N^2 rows (wall rows exit at once); all powers of two
     const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    // a block covers 2^log2rpb rows of 2^log2tpr threads each (rows shorter than
+    // a block), or one segment of 2^log2tpr threads of a row (gx segments)
+    const uint32_t seg = threadIdx.x >> a.log2tpr;                      // row of the block
+    const uint32_t tid = threadIdx.x & ((1u << a.log2tpr) - 1u);        // thread in its row segment
+    const uint32_t tpr = 1u << a.log2tpr;
     const uint32_t xb = lb & uint32_t(a.gx - 1);
-    const uint32_t row = lb >> a.log2gx;
+    const uint32_t row = ((lb >> a.log2gx) << a.log2rpb) + seg;
     const int32_t y = int32_t(row & uint32_t(n - 1));
     const int32_t z = int32_t(row >> a.log2n);
-    if (y == 0 || y == n - 1 || z == 0 || z == n - 1) {
+    const bool wall_row = (y == 0 || y == n - 1 || z == 0 || z == n - 1);  // uniform per wavefront
+    if (a.log2rpb == 0 && wall_row) {  // the whole block: nothing to do
         if (a.diagnostics && threadIdx.x == 0) {
             a.partials[2 * lb] = 0.0;
             a.partials[2 * lb + 1] = 0.0;
         }
         return;
     }
-    const uint32_t tid = threadIdx.x;
-    const int32_t x0 = int32_t((xb * blockDim.x + tid) * V);
+    const int32_t x0 = int32_t((xb * tpr + tid) * V);
     // first cell of this wavefront, made explicitly wavefront-uniform
-    const int32_t xw = __builtin_amdgcn_readfirstlane(int32_t((xb * blockDim.x + (tid & ~63u)) * V));
+    const int32_t xw = __builtin_amdgcn_readfirstlane(int32_t((xb * tpr + (tid & ~63u)) * V));
     const int64_t r = int64_t(y) + int64_t(n) * z;
     const int32_t rf = row_flags(y, z, n);
     const int64_t smask = a.stride - 1;
@@ -705,22 +718,28 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     // shifts.  Every slot is read by exactly one cell per step, so the loads
     // are streaming (nontemporal: 1.5-2.5% faster than cached loads at 512^3)
     SV A[Q];
+    if (!wall_row) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) A[i] = LBM_LAT_LOAD(reinterpret_cast<const SV*>(src_row(i) + off0));
-    // wavefront-end elements: from the neighbouring wavefront of the block
-    // through LDS, from memory only at the block's ends
+        for (int i = 0; i < Q; ++i) A[i] = LBM_LAT_LOAD(reinterpret_cast<const SV*>(src_row(i) + off0));
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) A[i] = SV{};
+    }
+    // wavefront-end elements: from the neighbouring wavefront of the row
+    // segment through LDS, from memory only at the segment's ends
     __shared__ S xe_up[16][5];  // per wavefront: lane 63's last element, e_x = +1 populations
     __shared__ S xe_dn[16][5];  // per wavefront: lane 0's first element, e_x = -1 populations
-    const uint32_t wave = tid >> 6, lane = tid & 63u, nwaves = blockDim.x >> 6;
+    const uint32_t wave = tid >> 6, lane = tid & 63u, nwaves = tpr >> 6;  // within the row segment
+    const uint32_t gw = threadIdx.x >> 6;                                  // within the block
     {
         int ku = 0, kd = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             if (ex_of(i) == 1) {
-                if (lane == 63) xe_up[wave][ku] = A[i][V - 1];
+                if (lane == 63) xe_up[gw][ku] = A[i][V - 1];
                 ++ku;
             } else if (ex_of(i) == -1) {
-                if (lane == 0) xe_dn[wave][kd] = A[i][0];
+                if (lane == 0) xe_dn[gw][kd] = A[i][0];
                 ++kd;
             }
         }
@@ -733,12 +752,12 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         for (int i = 0; i < Q; ++i) {
             edge[i] = S(0);
             if (ex_of(i) == 1) {
-                if (wave > 0) edge[i] = xe_up[wave - 1][ku];
-                else if (has_left) edge[i] = src_row(i)[voff(xw - 1)];
+                if (wave > 0) edge[i] = xe_up[gw - 1][ku];
+                else if (has_left && !wall_row) edge[i] = src_row(i)[voff(xw - 1)];
                 ++ku;
             } else if (ex_of(i) == -1) {
-                if (wave + 1 < nwaves) edge[i] = xe_dn[wave + 1][kd];
-                else if (has_right) edge[i] = src_row(i)[voff(xw + 64 * V)];
+                if (wave + 1 < nwaves) edge[i] = xe_dn[gw + 1][kd];
+                else if (has_right && !wall_row) edge[i] = src_row(i)[voff(xw + 64 * V)];
                 ++kd;
             }
         }
@@ -804,15 +823,17 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
 
     const int64_t oc0 = int64_t(n) * r;
     S* orow = dst + (19 * oc0 - 18 * (oc0 & smask));
+    if (!wall_row) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        SV o;
+        for (int i = 0; i < Q; ++i) {
+            SV o;
 #pragma unroll
-        for (int j = 0; j < V; ++j) o[j] = g[i][j];
-        LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+            for (int j = 0; j < V; ++j) o[j] = g[i][j];
+            LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+        }
     }
 
-    if (a.update_macro) {
+    if (a.update_macro && !wall_row) {
         // rho and u of the lane's V cells, in chunks of at most 16 bytes
         const int64_t c = int64_t(x0) + oc0;
         constexpr int CH = int(16 / sizeof(T)) < V ? int(16 / sizeof(T)) : V;
