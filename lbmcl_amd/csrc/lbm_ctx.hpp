@@ -343,6 +343,15 @@ inline int64_t row_block(const StepPlan* c) {
     return per_row < 256 ? per_row : 256;
 }
 inline int64_t row_gx(const StepPlan* c) { return (c->n / vec_width(c)) / row_block(c); }
+// Store policy of the vector kernels: lattices up to 384 MiB are stored through
+// the caches, so that much of what a step writes is still in the MI355X's
+// 256 MB memory-attached last-level cache when the next step reads it
+// (measured at 128^3: +13% FP32, +15% FP64); larger ones with streaming
+// stores (cached stores cost 6% at 512^3).
+inline bool cached_stores(const StepPlan* c) {
+    return double(c->n3) * lbm::Q * double(c->ssize) <= 384.0 * 1024 * 1024;
+}
+
 // Rows per block of the vector kernels: rows shorter than 256 threads share a
 // block (at most LBM_VEC_ROWS_PER_BLOCK of them), so short-row lattices launch
 // fewer, larger workgroups.
@@ -548,6 +557,7 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
         const unsigned bx = unsigned(row_block(c));
         if (kk == K_VEC || kk == K_VEC8) {
             const int64_t rpb = vec_rows_per_block(c);
+            a.cached_stores = cached_stores(c) ? 1 : 0;
             a.log2tpr = ilog2(int64_t(bx));
             a.log2rpb = ilog2(rpb);
             const dim3 block(unsigned(int64_t(bx) * rpb));

@@ -513,6 +513,7 @@ template <class T> struct StepArgs {
     int32_t log2gx;       // vector kernel: log2(gx)
     int32_t log2tpr;      // vector kernel: log2(threads per row segment of a block)
     int32_t log2rpb;      // vector kernel: log2(rows per block), > 0 for rows shorter than a block
+    int32_t cached_stores;  // vector kernel: store the lattice through the caches (small lattices)
     StepConsts<T> k;
     float scale, inv_scale;
     T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
@@ -824,12 +825,25 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     const int64_t oc0 = int64_t(n) * r;
     S* orow = dst + (19 * oc0 - 18 * (oc0 & smask));
     if (!wall_row) {
+        // streaming stores, or -- for lattices small enough to stay in the
+        // 256 MB memory-attached last-level cache until the next step reads
+        // them -- ordinary stores that allocate there
+        if (a.cached_stores) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            SV o;
+            for (int i = 0; i < Q; ++i) {
+                SV o;
 #pragma unroll
-            for (int j = 0; j < V; ++j) o[j] = g[i][j];
-            LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+                for (int j = 0; j < V; ++j) o[j] = g[i][j];
+                *reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0) = o;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                SV o;
+#pragma unroll
+                for (int j = 0; j < V; ++j) o[j] = g[i][j];
+                LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
+            }
         }
     }
 
