@@ -343,7 +343,7 @@ inline int64_t row_block(const StepPlan* c) {
     return per_row < 256 ? per_row : 256;
 }
 inline int64_t row_gx(const StepPlan* c) { return (c->n / vec_width(c)) / row_block(c); }
-// Store policy of the vector kernels: lattices up to 384 MiB are stored through
+// Store policy of the 8-byte-lane vector kernel: lattices up to 384 MiB are stored through
 // the caches, so that much of what a step writes is still in the MI355X's
 // 256 MB memory-attached last-level cache when the next step reads it
 // (measured at 128^3: +13% FP32, +15% FP64); larger ones with streaming
@@ -557,14 +557,15 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
         const unsigned bx = unsigned(row_block(c));
         if (kk == K_VEC || kk == K_VEC8) {
             const int64_t rpb = vec_rows_per_block(c);
-            a.cached_stores = cached_stores(c) ? 1 : 0;
             a.log2tpr = ilog2(int64_t(bx));
             a.log2rpb = ilog2(rpb);
             const dim3 block(unsigned(int64_t(bx) * rpb));
             if (kk == K_VEC)
                 launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, ARITH>, dim3(unsigned(nb)), block, e0, e1, a);
+            else if (cached_stores(c))
+                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH, true>, dim3(unsigned(nb)), block, e0, e1, a);
             else
-                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH>, dim3(unsigned(nb)), block, e0, e1, a);
+                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH, false>, dim3(unsigned(nb)), block, e0, e1, a);
         } else {
             launch_on(c, lbm::lbm_collide_stream_row<T, KIND, ARITH>, dim3(unsigned(nb)), dim3(bx), e0, e1, a);
         }

@@ -513,7 +513,6 @@ template <class T> struct StepArgs {
     int32_t log2gx;       // vector kernel: log2(gx)
     int32_t log2tpr;      // vector kernel: log2(threads per row segment of a block)
     int32_t log2rpb;      // vector kernel: log2(rows per block), > 0 for rows shorter than a block
-    int32_t cached_stores;  // vector kernel: store the lattice through the caches (small lattices)
     StepConsts<T> k;
     float scale, inv_scale;
     T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
@@ -667,7 +666,11 @@ template <class T, int KIND, int LB> struct VecWaves {
     static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? LBM_VEC8_NATIVE_WAVES : 3);
 };
 
-template <class T, int KIND, int LB, int ARITH>
+// CACHED: store the lattice through the caches (lattices small enough to stay
+// in the 256 MB memory-attached last-level cache until the next step reads
+// them), else with streaming stores.  A template parameter: a run-time branch
+// around the 19 stores cost 9% at 256^3.
+template <class T, int KIND, int LB, int ARITH, bool CACHED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VecWaves<T, KIND, LB>::value)))
 void lbm_collide_stream_vec(StepArgs<T> a) {
     using C = Codec<T, KIND>;
@@ -825,25 +828,14 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     const int64_t oc0 = int64_t(n) * r;
     S* orow = dst + (19 * oc0 - 18 * (oc0 & smask));
     if (!wall_row) {
-        // streaming stores, or -- for lattices small enough to stay in the
-        // 256 MB memory-attached last-level cache until the next step reads
-        // them -- ordinary stores that allocate there
-        if (a.cached_stores) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                SV o;
+        for (int i = 0; i < Q; ++i) {
+            SV o;
 #pragma unroll
-                for (int j = 0; j < V; ++j) o[j] = g[i][j];
-                *reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0) = o;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                SV o;
-#pragma unroll
-                for (int j = 0; j < V; ++j) o[j] = g[i][j];
-                LBM_LAT_STORE(o, reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0));
-            }
+            for (int j = 0; j < V; ++j) o[j] = g[i][j];
+            SV* p = reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0);
+            if constexpr (CACHED) *p = o;
+            else LBM_LAT_STORE(o, p);
         }
     }
 
