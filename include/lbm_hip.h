@@ -122,8 +122,13 @@ typedef struct lbm_config {
                                rounding of the reference's -D literals (for gradient checks) */
 } lbm_config;
 
-/* lbm_config cfg = LBM_CONFIG_INIT;  -- struct_size set, every other field zero */
+/* lbm_config cfg = LBM_CONFIG_INIT;  -- struct_size set, every other field zero
+ * (a designated initializer in C, so -Wmissing-field-initializers stays quiet) */
+#ifdef __cplusplus
 #define LBM_CONFIG_INIT {(uint32_t)sizeof(lbm_config)}
+#else
+#define LBM_CONFIG_INIT {.struct_size = (uint32_t)sizeof(lbm_config)}
+#endif
 
 /* objectives of lbm_adjoint (no counterpart in the reference) */
 #define LBM_OBJECTIVE_KINETIC_ENERGY 0  /* J = sum over FLUID/MOVING cells of rho |u|^2 / 2 */
@@ -137,7 +142,11 @@ typedef struct lbm_adjoint_config {
     int64_t checkpoint_every;  /* K: forward states kept every K steps; 0 = round(sqrt(T)) */
 } lbm_adjoint_config;
 
+#ifdef __cplusplus
 #define LBM_ADJOINT_CONFIG_INIT {(uint32_t)sizeof(lbm_adjoint_config)}
+#else
+#define LBM_ADJOINT_CONFIG_INIT {.struct_size = (uint32_t)sizeof(lbm_adjoint_config)}
+#endif
 
 typedef struct lbm_adjoint_result {
     double objective;     /* J */

@@ -4,6 +4,12 @@
       kernel trace (rocprofv3 --kernel-trace ... --output-format csv -d DIR):
       per kernel name and grid size, launches and average / min / max / total
       duration in ms.
+  python -m lbmcl_amd.rocprof_summary gaps DIR [--min-launches N]
+      the same trace split into bursts -- maximal runs of consecutive
+      dispatches of one kernel at one grid size, at least N (default 50) --
+      with, per burst, the average kernel duration, the average / median /
+      largest gap between one dispatch's end and the next one's start, and
+      the share of the burst's span spent inside the kernel.
   python -m lbmcl_amd.rocprof_summary counters DIR [DIR ...]
       counter collection (rocprofv3 --pmc ... -d DIR): per kernel name and grid
       size, the average value of every counter over its dispatches.
@@ -70,6 +76,35 @@ def trace(directory):
     return "\n".join(out)
 
 
+def gaps(directory, min_launches=50):
+    rows = []
+    for row in _rows(directory, "kernel_trace.csv"):
+        rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), _short(row["Kernel_Name"]), _grid(row)))
+    rows.sort()
+    bursts, cur = [], []
+    for r in rows:
+        if cur and (r[2], r[3]) != (cur[-1][2], cur[-1][3]):
+            bursts.append(cur)
+            cur = []
+        cur.append(r)
+    if cur:
+        bursts.append(cur)
+    out = ["| burst | kernel | grid | launches | avg kernel us | avg gap us | median gap us | max gap us | "
+           "kernel share of span |", "|---|---|---|---|---|---|---|---|---|"]
+    n = 0
+    for b in bursts:
+        if len(b) < min_launches:
+            continue
+        n += 1
+        dur = [(e - s) / 1e3 for s, e, _, _ in b]
+        gap = sorted((b[i + 1][0] - b[i][1]) / 1e3 for i in range(len(b) - 1))
+        span = (b[-1][1] - b[0][0]) / 1e3
+        out.append(f"| {n} | `{b[0][2]}` | {b[0][3]:,} | {len(b)} | {sum(dur) / len(dur):.2f} | "
+                   f"{sum(gap) / len(gap):.2f} | {gap[len(gap) // 2]:.2f} | {gap[-1]:.2f} | "
+                   f"{sum(dur) / span:.4f} |")
+    return "\n".join(out)
+
+
 def counters(directories):
     acc = defaultdict(lambda: defaultdict(list))
     for d in directories:
@@ -122,6 +157,9 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="what", required=True)
     t = sub.add_parser("trace")
     t.add_argument("dir")
+    g = sub.add_parser("gaps")
+    g.add_argument("dir")
+    g.add_argument("--min-launches", type=int, default=50)
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
     r = sub.add_parser("traffic")
@@ -132,6 +170,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.what == "trace":
         print(trace(a.dir))
+    elif a.what == "gaps":
+        print(gaps(a.dir, a.min_launches))
     elif a.what == "counters":
         print(counters(a.dirs)[0])
     else:

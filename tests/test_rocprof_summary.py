@@ -33,3 +33,17 @@ def test_trace_and_traffic(tmp_path):
     want = 4942000 * 1024 * 2 + 9884000 * 1024
     assert f"{want:.4e}" in out
     assert "kernel_sources_sha: " in out and f"traffic_bytes_per_launch: {want:.4e}" in out
+
+
+def test_gaps(tmp_path):
+    k, c = "void lbm::k<float>(args)", "copy_kernel(...)"
+    # burst 1: three launches of 10 us with gaps 2 and 4 us; one other kernel; burst 2: two launches, gap 1 us
+    rows = [[k, 256, 0, 10000], [k, 256, 12000, 22000], [k, 256, 26000, 36000], [c, 64, 40000, 41000],
+            [k, 256, 50000, 60000], [k, 256, 61000, 71000]]
+    _write(str(tmp_path / "t" / "run_kernel_trace.csv"), ["Kernel_Name", "Grid_Size", "Start_Timestamp",
+                                                           "End_Timestamp"], rows[::-1])
+    out = R.gaps(str(tmp_path / "t"), min_launches=2)
+    lines = [l for l in out.splitlines() if l.startswith("| 1") or l.startswith("| 2")]
+    assert lines[0] == "| 1 | `lbm::k<float>` | 256 | 3 | 10.00 | 3.00 | 4.00 | 4.00 | 0.8333 |"
+    assert lines[1] == "| 2 | `lbm::k<float>` | 256 | 2 | 10.00 | 1.00 | 1.00 | 1.00 | 0.9524 |"
+    assert len(lines) == 2  # the single copy dispatch is below min_launches
