@@ -211,6 +211,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 row")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="iterations of the 256^3 adjoint cost measurement (0: skip)")
+    ap.add_argument("--no-calibration", action="store_true",
+                    help="skip the copy and access-pattern legs (libmembw); used for runs under rocprofv3")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -265,9 +267,11 @@ def main():
         log(f"[bench] adjoint: forward {adjoint['forward_ms']:.0f} ms, adjoint {adjoint['adjoint_ms']:.0f} ms "
             f"({adjoint['adjoint_over_forward']:.2f}x), {adjoint['device_gib']:.1f} GiB")
 
-    log("[bench] copy bandwidth")
-    copy = copy_bandwidth()
-    pattern = access_pattern_bandwidth(stride) if dim == 512 else None
+    copy = pattern = None
+    if not args.no_calibration:
+        log("[bench] copy bandwidth")
+        copy = copy_bandwidth()
+        pattern = access_pattern_bandwidth(stride) if dim == 512 else None
     cpu = None
     if args.cpu_steps > 0 and rank == 0:
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle iterations at {dim}^3")

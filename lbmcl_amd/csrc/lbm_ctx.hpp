@@ -514,13 +514,15 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
     a.force = static_cast<const T*>(c->force);
     a.scale = float(c->scale);
     a.inv_scale = float(1.0 / c->scale);
+    a.k.dscale = a.scale;
+    a.k.dinv = a.inv_scale;
     {
         // lid-layer initial populations, same expression as the initialize kernel
         const T U = T(c->U);
         if constexpr (ARITH == lbm::AR_DELTA) {
             float d[lbm::Q];
             lbm::delta_lid_initial(float(c->U), d);
-            for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = T(d[i]);
+            for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = T(d[i] * float(c->scale));  // scaled, as the kernels hold them
         } else if constexpr (ARITH == lbm::AR_OPENCL) {
             const T u2 = lbm::u2_contracted<T>(U, T(0), T(0));
             for (int i = 0; i < lbm::Q; ++i) a.lid_eq[i] = lbm::feq_contracted<T>(i, T(1), U, T(0), T(0), u2);

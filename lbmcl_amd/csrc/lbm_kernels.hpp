@@ -75,6 +75,9 @@ template <> struct Codec<float, ST_FP16> {
     // deviation form: d = f - w_i itself, h = RN16(d * scale), d = h * (1/scale) (both products exact)
     __device__ __forceinline__ float decode_delta(S s) const { return float(s) * inv_scale; }
     __device__ __forceinline__ S encode_delta(float d) const { return _Float16(d * scale); }
+    // the collision's own domain: the scaled deviation d * scale, exactly the stored value
+    __device__ __forceinline__ float raw(S s) const { return float(s); }
+    __device__ __forceinline__ S from_raw(float v) const { return _Float16(v); }
 };
 
 template <> struct Codec<float, ST_BF16> {
@@ -92,6 +95,8 @@ template <> struct Codec<float, ST_BF16> {
     __device__ __forceinline__ void store(S* p, float w, float v) const { __builtin_nontemporal_store(encode(v, w), p); }
     __device__ __forceinline__ float decode_delta(S s) const { return __uint_as_float(uint32_t(s) << 16) * inv_scale; }
     __device__ __forceinline__ S encode_delta(float d) const { return __builtin_bit_cast(uint16_t, __bf16(d * scale)); }
+    __device__ __forceinline__ float raw(S s) const { return __uint_as_float(uint32_t(s) << 16); }
+    __device__ __forceinline__ S from_raw(float v) const { return __builtin_bit_cast(uint16_t, __bf16(v)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -127,6 +132,7 @@ template <class T> struct StepConsts {
     T inv_tau;  // 1 / (3 nu + 1/2) folded in T (kernels.cl:61-62)
     T U;        // lid velocity literal (kernels.cl:57)
     T om1;      // 1 - inv_tau, for the fused collision
+    float dscale, dinv;  // deviation form: storage scale (a power of two) and its inverse
 };
 
 // e_i . u with the reference's ((ux ex + uy ey) + uz ez) evaluated exactly:
@@ -350,6 +356,14 @@ __device__ __forceinline__ void add_force(int32_t t, T (&f)[Q], T gx, T gy, T gz
 // tau shared between opposite populations as A +- 3e, A = 4.5 e e + Km1; the
 // lid sets d_i = w_i fma(rho, tau_i, drho).  The CPU oracle evaluates the same
 // operations in the same order.
+//
+// The kernels keep the deviations scaled, h_i = d_i * scale -- the stored
+// values as they are -- and fold the power-of-two scale into the constants:
+// drho = (sum h_i) / scale, j = (sum e_i h_i) / scale, h_i' = fma(cw_i scale,
+// fma(rho, tau_i, drho), (1 - w) h_i).  Multiplying by a power of two commutes
+// with every rounding here (all values are far from FP32's subnormal and
+// overflow ranges), so these are the oracle's unscaled results times scale,
+// bit for bit, at two multiplications per population less.
 // ---------------------------------------------------------------------------
 // d_i <- fma(cw_i, fma(rho, tau_i, drho), keep * d_i); cw_i = c0, c1, c7 by |e_i|^2
 __host__ __device__ __forceinline__ void delta_relax(float rho, float drho, float ux, float uy, float uz, float c0,
@@ -391,9 +405,10 @@ __device__ __forceinline__ void collide_delta(int32_t t, T (&d)[Q], T& rho, T& u
         d[13] = d[15];
         d[14] = d[16];
     }
-    T drho = d[0];
+    T hrho = d[0];  // d[] holds scaled deviations h_i = d_i * scale
 #pragma unroll
-    for (int i = 1; i < Q; ++i) drho = drho + d[i];
+    for (int i = 1; i < Q; ++i) hrho = hrho + d[i];
+    const T drho = hrho * k.dinv;
     rho = T(1) + drho;
     if (moving) {  // kernels.cl:362-365
         ux = k.U;
@@ -403,19 +418,19 @@ __device__ __forceinline__ void collide_delta(int32_t t, T (&d)[Q], T& rho, T& u
         const T jx = ((((d[1] + d[7]) + d[10]) + d[11]) + d[15]) - ((((d[3] + d[8]) + d[9]) + d[13]) + d[17]);
         const T jy = ((((d[2] + d[7]) + d[8]) + d[12]) + d[16]) - ((((d[4] + d[9]) + d[10]) + d[14]) + d[18]);
         const T jz = ((((d[6] + d[15]) + d[16]) + d[17]) + d[18]) - ((((d[5] + d[11]) + d[12]) + d[13]) + d[14]);
-        ux = jx / rho;
-        uy = jy / rho;
-        uz = jz / rho;
+        ux = (jx * k.dinv) / rho;
+        uy = (jy * k.dinv) / rho;
+        uz = (jz * k.dinv) / rho;
     }
     if (!moving && t != FLUID) {
         if (is_bounceback(t)) bounce_back<T>(d);
         return;  // CORNER: pure propagation
     }
     if (moving)
-        delta_relax(rho, drho, ux, uy, uz, W<T>::w0, W<T>::w1, W<T>::w7, T(0), d);
+        delta_relax(rho, drho, ux, uy, uz, W<T>::w0 * k.dscale, W<T>::w1 * k.dscale, W<T>::w7 * k.dscale, T(0), d);
     else
-        delta_relax(rho, drho, ux, uy, uz, k.inv_tau * W<T>::w0, k.inv_tau * W<T>::w1, k.inv_tau * W<T>::w7, k.om1,
-                    d);
+        delta_relax(rho, drho, ux, uy, uz, (k.inv_tau * W<T>::w0) * k.dscale, (k.inv_tau * W<T>::w1) * k.dscale,
+                    (k.inv_tau * W<T>::w7) * k.dscale, k.om1, d);
 }
 
 template <class T, int ARITH>
@@ -427,17 +442,24 @@ __device__ __forceinline__ void collide_any(int32_t t, T (&f)[Q], T& rho, T& ux,
     else collide<T>(t, f, rho, ux, uy, uz, k);
 }
 
-// Storage <-> collision values: the population, or its deviation from w_i in
-// the deviation form.
+// Storage <-> collision values: the population, or in the deviation form its
+// scaled deviation from w_i (the stored value itself).
 template <int ARITH, class C, class S, class T>
 __device__ __forceinline__ T value_in(const C& codec, S s, T w) {
-    if constexpr (ARITH == AR_DELTA) return codec.decode_delta(s);
+    if constexpr (ARITH == AR_DELTA) return codec.raw(s);
     else return codec.decode(s, w);
 }
 template <int ARITH, class C, class T>
 __device__ __forceinline__ typename C::S value_out(const C& codec, T v, T w) {
-    if constexpr (ARITH == AR_DELTA) return codec.encode_delta(v);
+    if constexpr (ARITH == AR_DELTA) return codec.from_raw(v);
     else return codec.encode(v, w);
+}
+
+// Body force on collision values: 3 w_i (e_i . g), scaled in the deviation form.
+template <class T, int ARITH>
+__device__ __forceinline__ void add_force_any(int32_t t, T (&f)[Q], T gx, T gy, T gz, const StepConsts<T>& k) {
+    if constexpr (ARITH == AR_DELTA) add_force<T>(t, f, gx * k.dscale, gy * k.dscale, gz * k.dscale);
+    else add_force<T>(t, f, gx, gy, gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,7 +537,8 @@ template <class T> struct StepArgs {
     int32_t log2rpb;      // vector kernel: log2(rows per block), > 0 for rows shorter than a block
     StepConsts<T> k;
     float scale, inv_scale;
-    T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells
+    T lid_eq[Q];          // feq_i(1, (U, 0, 0)): initial populations of lid-layer cells (deviation form:
+                          // their deviations from w_i times the storage scale)
     const T* force;       // body force g, 3 N^3 [gx | gy | gz], or null
 };
 
@@ -578,7 +601,7 @@ __global__ __launch_bounds__(256) void lbm_collide_stream_row(StepArgs<T> a) {
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         if (a.force) {
             const int64_t c = int64_t(x) + int64_t(n) * r;
-            add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+            add_force_any<T, ARITH>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c], a.k);
         }
         const int64_t c0 = int64_t(n) * r;
         const int64_t obase = 19 * c0 - 18 * (c0 & smask);
@@ -647,8 +670,8 @@ template <class S> __device__ __forceinline__ S lane_shift(S v, S edge, bool fro
 // Initial pull-form value of population i at the wall cell x = 0 (side -1) or
 // x = N-1 (side +1) of row (y, z): F0 of the destination (x + e_x, y + e_y,
 // z + e_z), i.e. NaN unless the destination is a fluid-side cell.
-// In the deviation form (ARITH == AR_DELTA) the value is F0 - w_i: lid_eq then
-// holds the lid-layer deviations and the rest value is 0.
+// In the deviation form (ARITH == AR_DELTA) the value is (F0 - w_i) * scale:
+// lid_eq then holds the scaled lid-layer deviations and the rest value is 0.
 template <class T, int ARITH = AR_EXACT>
 __device__ __forceinline__ T wall_p0(int i, int side, int32_t y, int32_t z, int32_t n, const T* lid_eq) {
     const int32_t dy = y + ey_of(i), dz = z + ez_of(i);
@@ -797,7 +820,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
         if (a.force) {
             const int64_t c = int64_t(x0 + j) + int64_t(n) * r;
-            add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+            add_force_any<T, ARITH>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c], a.k);
         }
         const bool store = is_store_macro(t);
         const T qnan = T(__builtin_nan(""));
@@ -1015,7 +1038,7 @@ __global__ __launch_bounds__((LdsRow<typename Codec<T, KIND>::S, NX>::THREADS)) 
             collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);  // WALL: passes f through unchanged
             if (a.force) {
                 const int64_t cc = int64_t(x0 + c) + int64_t(n) * r;
-                add_force<T>(t, f, a.force[cc], a.force[a.n3 + cc], a.force[2 * a.n3 + cc]);
+                add_force_any<T, ARITH>(t, f, a.force[cc], a.force[a.n3 + cc], a.force[2 * a.n3 + cc], a.k);
             }
             const bool store = is_store_macro(t);
             if (a.diagnostics && store) {
@@ -1210,7 +1233,7 @@ __global__ __launch_bounds__((TileThreads<T, KIND>::value)) void lbm_collide_str
         }
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
         const int64_t c = int64_t(x) + int64_t(n) * (int64_t(y) + int64_t(n) * z);
-        if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+        if (a.force) add_force_any<T, ARITH>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c], a.k);
         if (a.update_macro && is_store_macro(t)) {
             a.rho[c] = rho;
             a.u[c] = ux;
@@ -1283,7 +1306,7 @@ __global__ __launch_bounds__(1024) void lbm_collide_stream_generic(StepArgs<T> a
             f[i] = value_in<ARITH>(codec, src[slot(s, i, a.stride)], w_of<T>(i));
         }
         collide_any<T, ARITH>(t, f, rho, ux, uy, uz, a.k);
-        if (a.force) add_force<T>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c]);
+        if (a.force) add_force_any<T, ARITH>(t, f, a.force[c], a.force[a.n3 + c], a.force[2 * a.n3 + c], a.k);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
             __builtin_nontemporal_store(value_out<ARITH>(codec, f[i], w_of<T>(i)), dst + slot(c, i, a.stride));

@@ -13,12 +13,17 @@
   python -m lbmcl_amd.rocprof_summary counters DIR [DIR ...]
       counter collection (rocprofv3 --pmc ... -d DIR): per kernel name and grid
       size, the average value of every counter over its dispatches.
-  python -m lbmcl_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR
+  python -m lbmcl_amd.rocprof_summary calibrate
+      the calibration workload, run under rocprofv3 --pmc: 2^30 FP32 values
+      multiplied by 2 into a second array, five times (torch's elementwise
+      kernel: exactly 4 GiB read and 4 GiB written per dispatch).
+  python -m lbmcl_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR [--calibration FETCH_DIR WRITE_DIR]
       HBM traffic per dispatch from FETCH_SIZE and WRITE_SIZE (KiB), scaled by
-      what the same counters read for bench.py's copy kernel, which moves
-      exactly 4 GiB each way; also prints the kernel sources' hash
-      (`kernel_sources_sha`) that bench.py compares before it reports the
-      traffic of its headline kernel.
+      what the same counters read for a kernel that moves a known number of
+      bytes: the `calibrate` workload (its own runs, given by --calibration)
+      or else bench.py's copy kernel (4 GiB each way) in the same runs; also
+      prints the kernel sources' hash (`kernel_sources_sha`) that bench.py
+      compares before it reports the traffic of its headline kernel.
 
 Markdown tables on stdout.  The CSV columns are found by name, so the files of
 every rocprofv3 version that writes Kernel_Name / Grid_Size / Start_Timestamp /
@@ -33,7 +38,8 @@ import os
 import sys
 from collections import defaultdict
 
-COPY_BYTES = 4 << 30  # bench.py's copy kernel: 4 GiB read and 4 GiB written per dispatch
+COPY_BYTES = 4 << 30  # the calibration kernels: 4 GiB read and 4 GiB written per dispatch
+CALIBRATION_KERNELS = ("copy_kernel", "MulFunctor")  # bench.py's membw copy; the `calibrate` workload
 
 
 def _rows(directory, suffix):
@@ -57,11 +63,19 @@ def _grid(row):
 
 
 def _short(name):
-    """The kernel's name without `void ` and its parameter list."""
-    name = name.strip()
+    """The kernel's name without `void `, `(anonymous namespace)::` and its parameter list."""
+    name = name.strip().replace("(anonymous namespace)::", "")
     if name.startswith("void "):
         name = name[5:]
-    return name.split("(")[0]
+    depth = 0
+    for i, ch in enumerate(name):  # cut at the parameter list, not inside template arguments
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
 
 
 def trace(directory):
@@ -118,20 +132,31 @@ def counters(directories):
     return "\n".join(out), acc
 
 
-def traffic(fetch_dir, write_dir, headline=None):
+def _calibration_value(acc, counter, what):
+    keys = [k for k in acc if any(c in k[0] for c in CALIBRATION_KERNELS) and counter in acc[k]]
+    if not keys:
+        raise SystemExit(f"no calibration kernel ({' or '.join(CALIBRATION_KERNELS)}) in the {what} run; kernels: "
+                         + ", ".join(sorted({k[0][:60] for k in acc})))
+    key = max(keys, key=lambda k: k[1])  # the 4 GiB dispatch (torch's fill of the input is not matched)
+    v = acc[key][counter]
+    return key[0], sum(v) / len(v)
+
+
+def traffic(fetch_dir, write_dir, headline=None, calibration=None):
     """headline: (kernel name, grid) whose traffic bench.py reports; printed as
-    `traffic_bytes_per_launch:` next to the sources' hash."""
+    `traffic_bytes_per_launch:` next to the sources' hash.  calibration:
+    (fetch dir, write dir) of the `calibrate` workload, else the copy kernel
+    is taken from the runs themselves."""
     _, f = counters([fetch_dir])
     _, w = counters([write_dir])
-    copy = [k for k in f if k[0].endswith("copy_kernel")]
-    if not copy:
-        raise SystemExit("the copy kernel of bench.py is not in the fetch run")
-    cf = sum(f[copy[0]]["FETCH_SIZE"]) / len(f[copy[0]]["FETCH_SIZE"])
-    cw_key = [k for k in w if k[0].endswith("copy_kernel")][0]
-    cw = sum(w[cw_key]["WRITE_SIZE"]) / len(w[cw_key]["WRITE_SIZE"])
+    _, cfa = counters([calibration[0]]) if calibration else (None, f)
+    _, cwa = counters([calibration[1]]) if calibration else (None, w)
+    cname, cf = _calibration_value(cfa, "FETCH_SIZE", "fetch")
+    _, cw = _calibration_value(cwa, "WRITE_SIZE", "write")
     sf, sw = COPY_BYTES / (cf * 1024), COPY_BYTES / (cw * 1024)
-    out = [f"Calibration: the copy kernel (4 GiB each way) reads FETCH_SIZE {cf:,.0f} KiB and WRITE_SIZE "
-           f"{cw:,.0f} KiB per dispatch, so bytes = FETCH_SIZE x 1024 x {sf:.4f} and WRITE_SIZE x 1024 x {sw:.4f}.",
+    out = [f"Calibration: `{cname[:80]}` (4 GiB read and 4 GiB written per dispatch) reads FETCH_SIZE {cf:,.0f} "
+           f"KiB and WRITE_SIZE {cw:,.0f} KiB per dispatch, so bytes = FETCH_SIZE x 1024 x {sf:.4f} and "
+           f"WRITE_SIZE x 1024 x {sw:.4f}.",
            "", "| kernel | grid | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic (B) |", "|---|---|---|---|---|"]
     head = None
     for key in sorted(f):
@@ -152,6 +177,17 @@ def traffic(fetch_dir, write_dir, headline=None):
     return "\n".join(out)
 
 
+def calibrate(reps=5):
+    import torch
+    a = torch.full((1 << 30,), 1.5, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    assert float(b[-1]) == 3.0
+    print(f"calibrate: {reps} x (4 GiB read, 4 GiB written)")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="what", required=True)
@@ -162,9 +198,12 @@ def main(argv=None):
     g.add_argument("--min-launches", type=int, default=50)
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
+    sub.add_parser("calibrate")
     r = sub.add_parser("traffic")
     r.add_argument("fetch_dir")
     r.add_argument("write_dir")
+    r.add_argument("--calibration", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                   help="FETCH_SIZE and WRITE_SIZE runs of the `calibrate` workload")
     r.add_argument("--headline", nargs=2, metavar=("KERNEL", "GRID"),
                    help="kernel name (as in the table) and grid of bench.py's headline kernel")
     a = ap.parse_args(argv)
@@ -172,11 +211,13 @@ def main(argv=None):
         print(trace(a.dir))
     elif a.what == "gaps":
         print(gaps(a.dir, a.min_launches))
+    elif a.what == "calibrate":
+        calibrate()
     elif a.what == "counters":
         print(counters(a.dirs)[0])
     else:
         head = (a.headline[0], int(a.headline[1])) if a.headline else None
-        print(traffic(a.fetch_dir, a.write_dir, head))
+        print(traffic(a.fetch_dir, a.write_dir, head, a.calibration))
     return 0
 
 
