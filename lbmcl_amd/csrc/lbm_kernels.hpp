@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "lbm_lattice.hpp"
 
@@ -685,6 +686,23 @@ __device__ __forceinline__ T wall_p0(int i, int side, int32_t y, int32_t z, int3
 #ifndef LBM_VEC8_NATIVE_WAVES
 #define LBM_VEC8_NATIVE_WAVES 4
 #endif
+// base + uoff + off elements as (uniform address) + (32-bit byte offset): the
+// form the global load/store instructions take with the address in scalar
+// registers (saddr), so no 64-bit per-lane address arithmetic.  uoff is
+// wavefront-uniform; it goes through readfirstlane, which keeps the compiler
+// from folding the per-lane offset into it (bases that differ by multiples of
+// the stride were otherwise turned into one per-lane base plus 64-bit
+// multiply-adds per access).  off * sizeof(E) must be below 2^32 (row offsets
+// here are below 19 * 1024 elements).
+template <class P, class E>
+__device__ __forceinline__ P* at_offset(E* base, int64_t uoff, uint32_t off) {
+    using B = typename std::conditional<std::is_const<E>::value, const char, char>::type;
+    const uint64_t v = uint64_t(uoff);
+    const int64_t u = int64_t(uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v)))) |
+                              (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v >> 32)))) << 32));
+    return reinterpret_cast<P*>(reinterpret_cast<B*>(base + u) + off * uint32_t(sizeof(E)));
+}
+
 template <class T, int KIND, int LB> struct VecWaves {
     static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? LBM_VEC8_NATIVE_WAVES : 3);
 };
@@ -706,7 +724,9 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x, a.xcd_remap);
     // a block covers 2^log2rpb rows of 2^log2tpr threads each (rows shorter than
     // a block), or one segment of 2^log2tpr threads of a row (gx segments)
-    const uint32_t seg = threadIdx.x >> a.log2tpr;                      // row of the block
+    // row of the block: wavefront-uniform (a row segment holds whole wavefronts,
+    // 2^log2tpr >= 64), made explicit so the row's addresses stay scalar
+    const uint32_t seg = __builtin_amdgcn_readfirstlane(threadIdx.x >> a.log2tpr);
     const uint32_t tid = threadIdx.x & ((1u << a.log2tpr) - 1u);        // thread in its row segment
     const uint32_t tpr = 1u << a.log2tpr;
     const uint32_t xb = lb & uint32_t(a.gx - 1);
@@ -732,10 +752,12 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
         const uint32_t u = uint32_t(xx);
         return 19u * u - 18u * (u & smask32);
     };
-    auto src_row = [&](int i) -> const S* {  // slot 0 of the source row of population i
+    auto src_off = [&](int i) -> int64_t {  // slot 0 of the source row of population i
         const int64_t c0 = int64_t(n) * (r - ey_of(i) - int64_t(n) * ez_of(i));
-        return static_cast<const S*>(a.src) + (19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride);
+        return 19 * c0 - 18 * (c0 & smask) + int64_t(i) * a.stride;
     };
+    const S* __restrict__ src = static_cast<const S*>(a.src);
+    auto src_row = [&](int i) -> const S* { return src + src_off(i); };
     const uint32_t off0 = voff(x0);
     const bool has_left = xw > 0;               // cell xw-1 exists (it is not needed when xw = 0)
     const bool has_right = xw + 64 * V < n;     // cell xw+64V exists
@@ -747,7 +769,7 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     SV A[Q];
     if (!wall_row) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) A[i] = LBM_LAT_LOAD(reinterpret_cast<const SV*>(src_row(i) + off0));
+        for (int i = 0; i < Q; ++i) A[i] = LBM_LAT_LOAD(at_offset<const SV>(src, src_off(i), off0));
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) A[i] = SV{};
@@ -849,14 +871,14 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     }
 
     const int64_t oc0 = int64_t(n) * r;
-    S* orow = dst + (19 * oc0 - 18 * (oc0 & smask));
+    const int64_t orow = 19 * oc0 - 18 * (oc0 & smask);
     if (!wall_row) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             SV o;
 #pragma unroll
             for (int j = 0; j < V; ++j) o[j] = g[i][j];
-            SV* p = reinterpret_cast<SV*>(orow + int64_t(i) * a.stride + off0);
+            SV* p = at_offset<SV>(dst, orow + int64_t(i) * a.stride, off0);
             if constexpr (CACHED) *p = o;
             else LBM_LAT_STORE(o, p);
         }
