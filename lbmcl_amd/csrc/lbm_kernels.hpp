@@ -703,8 +703,12 @@ __device__ __forceinline__ P* at_offset(E* base, int64_t uoff, uint32_t off) {
     return reinterpret_cast<P*>(reinterpret_cast<B*>(base + u) + off * uint32_t(sizeof(E)));
 }
 
+#ifndef LBM_VEC8_16BIT_WAVES
+#define LBM_VEC8_16BIT_WAVES 3
+#endif
 template <class T, int KIND, int LB> struct VecWaves {
-    static constexpr int value = (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? LBM_VEC8_NATIVE_WAVES : 3);
+    static constexpr int value =
+        (LB == 16) ? (sizeof(T) == 8 ? 3 : 2) : (KIND == ST_NATIVE ? LBM_VEC8_NATIVE_WAVES : LBM_VEC8_16BIT_WAVES);
 };
 
 // CACHED: store the lattice through the caches (lattices small enough to stay
