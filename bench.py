@@ -110,7 +110,8 @@ def access_pattern_bandwidth(stride, reps=6):
     """The FP32 step's data movement at 512^3 without its arithmetic (membw.hip
     pattern_kernel): algorithmic GB/s (152 B per wet cell) at full occupancy (8
     waves per SIMD), at five -- the FP32 vector kernel's occupancy (88 VGPRs) --
-    at four and at two (32, 40 and 64 KB of LDS per 4-wave block)."""
+    at four and at two (30, 40 and 64 KiB of LDS per 4-wave block; 160 KiB per
+    CU)."""
     path = os.path.join(ROOT, "lbmcl_amd", "libmembw.so")
     if not os.path.exists(path):
         return None
@@ -118,7 +119,7 @@ def access_pattern_bandwidth(stride, reps=6):
     L.membw_pattern.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_double)]
     out = {"dim": 512, "stride": stride, "bytes_per_cell": 152}
-    for name, lds in (("full_occupancy", 0), ("five_waves_per_simd", 32768), ("four_waves_per_simd", 40960),
+    for name, lds in (("full_occupancy", 0), ("five_waves_per_simd", 30720), ("four_waves_per_simd", 40960),
                       ("two_waves_per_simd", 65536)):
         ms = (ctypes.c_double * reps)()
         if L.membw_pattern(0, 512, stride, lds, reps, ms) != 0:

@@ -4,7 +4,8 @@ Outputs (git-ignored, they travel with the tree to the GPU host):
   lbmcl_amd/liblbm_hip.so      HIP kernels + C ABI (include/lbm_hip.h), gfx950
   lbmcl_amd/bin/lbmcl          the lbmcl command line (C++ host, links liblbm_hip.so)
   lbmcl_amd/bin/capi_example   the C ABI used from plain C (csrc/examples/capi_example.c)
-  lbmcl_amd/libmembw.so        plain HBM copy kernel used to calibrate bandwidth figures
+  lbmcl_amd/libmembw.so        copy and access-pattern calibration kernels (bench.py)
+  lbmcl_amd/bin/membw          the same as a program of its own (rocprofv3 counter calibration)
 
 Usage: python -m lbmcl_amd.build [--only lib|cli|membw]
 """
@@ -35,6 +36,7 @@ HIP_FLAGS = [
 LIB = os.path.join(PKG, "liblbm_hip.so")
 MEMBW = os.path.join(PKG, "libmembw.so")
 CLI = os.path.join(BIN, "lbmcl")
+MEMBW_BIN = os.path.join(BIN, "membw")
 CAPI_EXAMPLE = os.path.join(BIN, "capi_example")
 
 
@@ -60,6 +62,8 @@ def build_lib():
 
 def build_membw():
     _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "membw.hip"), "-o", MEMBW])
+    os.makedirs(BIN, exist_ok=True)
+    _run([HIPCC, *HIP_FLAGS, "-DMEMBW_MAIN", os.path.join(CSRC, "membw.hip"), "-o", MEMBW_BIN])
 
 
 def build_cli():

@@ -784,7 +784,9 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
     __shared__ S xe_dn[16][5];  // per wavefront: lane 0's first element, e_x = -1 populations
     const uint32_t wave = tid >> 6, lane = tid & 63u, nwaves = tpr >> 6;  // within the row segment
     const uint32_t gw = threadIdx.x >> 6;                                  // within the block
-    {
+    // Rows of one wavefront (nwaves == 1, block-uniform) exchange nothing: their
+    // ends come from memory, and their block -- several such rows -- needs no barrier.
+    if (nwaves > 1) {
         int ku = 0, kd = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -796,8 +798,8 @@ void lbm_collide_stream_vec(StepArgs<T> a) {
                 ++kd;
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
     S edge[Q];
     {
         int ku = 0, kd = 0;

@@ -148,3 +148,35 @@ int membw_copy(int device, int64_t bytes, int reps, double* ms) {
 }
 
 }  // extern "C"
+
+#ifdef MEMBW_MAIN
+// Stand-alone form for counter calibration (rocprofv3 --pmc on a process that
+// runs nothing else):  membw copy           -- five 4 GiB copies
+//                      membw pattern LDS    -- six runs of the 512^3 FP32 step's
+//                                              access pattern, LDS bytes per block
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+int main(int argc, char** argv) {
+    double ms[6] = {};
+    if (argc >= 2 && !std::strcmp(argv[1], "copy")) {
+        const int64_t bytes = int64_t(4) << 30;
+        const int rc = membw_copy(0, bytes, 5, ms);
+        if (rc) return std::fprintf(stderr, "membw_copy failed (%d)\n", rc), 1;
+        for (int r = 0; r < 5; ++r) std::printf("copy 4 GiB: %.4f ms, %.1f GB/s\n", ms[r], 2.0 * bytes / (ms[r] * 1e6));
+        return 0;
+    }
+    if (argc >= 3 && !std::strcmp(argv[1], "pattern")) {
+        const int lds = std::atoi(argv[2]);
+        const int rc = membw_pattern(0, 512, 4096, lds, 6, ms);
+        if (rc) return std::fprintf(stderr, "membw_pattern failed (%d)\n", rc), 1;
+        for (int r = 0; r < 6; ++r)
+            std::printf("pattern 512^3 stride 4096 lds %d: %.4f ms, %.1f GB/s\n", lds, ms[r],
+                        510.0 * 510.0 * 510.0 * 152.0 / (ms[r] * 1e6));
+        return 0;
+    }
+    std::fprintf(stderr, "usage: membw copy | membw pattern LDS_BYTES\n");
+    return 2;
+}
+#endif

@@ -13,17 +13,17 @@
   python -m lbmcl_amd.rocprof_summary counters DIR [DIR ...]
       counter collection (rocprofv3 --pmc ... -d DIR): per kernel name and grid
       size, the average value of every counter over its dispatches.
-  python -m lbmcl_amd.rocprof_summary calibrate
-      the calibration workload, run under rocprofv3 --pmc: 2^30 FP32 values
-      multiplied by 2 into a second array, five times (torch's elementwise
-      kernel: exactly 4 GiB read and 4 GiB written per dispatch).
   python -m lbmcl_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR [--calibration FETCH_DIR WRITE_DIR]
       HBM traffic per dispatch from FETCH_SIZE and WRITE_SIZE (KiB), scaled by
-      what the same counters read for a kernel that moves a known number of
-      bytes: the `calibrate` workload (its own runs, given by --calibration)
-      or else bench.py's copy kernel (4 GiB each way) in the same runs; also
-      prints the kernel sources' hash (`kernel_sources_sha`) that bench.py
-      compares before it reports the traffic of its headline kernel.
+      what the same counters read for membw's copy kernel (4 GiB read and 4 GiB
+      written per dispatch, 16-byte streaming loads and stores like the
+      collide-stream kernels' accesses), from runs of `lbmcl_amd/bin/membw
+      copy` given by --calibration, or else from the same runs; also prints
+      the kernel sources' hash (`kernel_sources_sha`) that bench.py compares
+      before it reports the traffic of its headline kernel.  (The counters'
+      reading depends on the kind of request: torch's elementwise kernels,
+      with ordinary loads, read half of what this copy reads per byte, so the
+      calibration kernel has to access memory the way the measured kernels do.)
 
 Markdown tables on stdout.  The CSV columns are found by name, so the files of
 every rocprofv3 version that writes Kernel_Name / Grid_Size / Start_Timestamp /
@@ -39,7 +39,7 @@ import sys
 from collections import defaultdict
 
 COPY_BYTES = 4 << 30  # the calibration kernels: 4 GiB read and 4 GiB written per dispatch
-CALIBRATION_KERNELS = ("copy_kernel", "MulFunctor")  # bench.py's membw copy; the `calibrate` workload
+CALIBRATION_KERNELS = ("copy_kernel",)  # membw.hip's copy
 
 
 def _rows(directory, suffix):
@@ -137,7 +137,7 @@ def _calibration_value(acc, counter, what):
     if not keys:
         raise SystemExit(f"no calibration kernel ({' or '.join(CALIBRATION_KERNELS)}) in the {what} run; kernels: "
                          + ", ".join(sorted({k[0][:60] for k in acc})))
-    key = max(keys, key=lambda k: k[1])  # the 4 GiB dispatch (torch's fill of the input is not matched)
+    key = max(keys, key=lambda k: k[1])  # the 4 GiB dispatch
     v = acc[key][counter]
     return key[0], sum(v) / len(v)
 
@@ -177,17 +177,6 @@ def traffic(fetch_dir, write_dir, headline=None, calibration=None):
     return "\n".join(out)
 
 
-def calibrate(reps=5):
-    import torch
-    a = torch.full((1 << 30,), 1.5, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        torch.mul(a, 2.0, out=b)
-    torch.cuda.synchronize()
-    assert float(b[-1]) == 3.0
-    print(f"calibrate: {reps} x (4 GiB read, 4 GiB written)")
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="what", required=True)
@@ -198,12 +187,11 @@ def main(argv=None):
     g.add_argument("--min-launches", type=int, default=50)
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
-    sub.add_parser("calibrate")
     r = sub.add_parser("traffic")
     r.add_argument("fetch_dir")
     r.add_argument("write_dir")
     r.add_argument("--calibration", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                   help="FETCH_SIZE and WRITE_SIZE runs of the `calibrate` workload")
+                   help="FETCH_SIZE and WRITE_SIZE runs of `lbmcl_amd/bin/membw copy`")
     r.add_argument("--headline", nargs=2, metavar=("KERNEL", "GRID"),
                    help="kernel name (as in the table) and grid of bench.py's headline kernel")
     a = ap.parse_args(argv)
@@ -211,8 +199,6 @@ def main(argv=None):
         print(trace(a.dir))
     elif a.what == "gaps":
         print(gaps(a.dir, a.min_launches))
-    elif a.what == "calibrate":
-        calibrate()
     elif a.what == "counters":
         print(counters(a.dirs)[0])
     else:
