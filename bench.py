@@ -210,7 +210,7 @@ def main():
                     help="iterations of the CPU oracle sample, ~10 s on 16 threads (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-remap", action="store_true", help="disable the XCD-aware block order")
-    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 row")
+    ap.add_argument("--kernel", type=int, default=0, help="lbm_config.kernel: 0 auto, 1 generic, 2 row, 3 vector (16-byte lanes), 4 vector (8-byte), 5 tile, 6 row pipeline")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="iterations of the 256^3 adjoint cost measurement (0: skip)")
     ap.add_argument("--no-calibration", action="store_true",

@@ -83,7 +83,7 @@ extern "C" {
  * reference's results on this GPU, bit for bit); 16-bit storage has no
  * counterpart in the reference and always computes in its own deviation form
  * (see lbm_compute), whatever optimize says (LBM_ARITH_EXACT is rejected). */
-#define LBM_ARITH_REFERENCE 0 /* default: LBM_ARITH_OPENCL for native storage, LBM_ARITH_FUSED for 16-bit */
+#define LBM_ARITH_REFERENCE 0 /* default: LBM_ARITH_OPENCL for native storage; 16-bit: the deviation form */
 #define LBM_ARITH_OPENCL 0    /* kernels.cl's expressions as ROCm's OpenCL compiler evaluates them */
 #define LBM_ARITH_FUSED 1     /* -o: the fused BGK form (native storage) */
 #define LBM_ARITH_EXACT 2     /* the CPU oracle's form: kernels.cl's expressions operation by operation,

@@ -29,7 +29,7 @@ import numpy as np
 
 from . import capi
 
-# (precision, storage, arithmetic); 16-bit storage always computes in the fused form
+# (precision, storage, arithmetic); 16-bit storage always computes in the deviation form
 FORMS = (("f32", "native", "reference"), ("f32", "native", "fused"), ("f32", "fp16", "reference"),
          ("f32", "bf16", "reference"))
 

@@ -27,7 +27,7 @@ LBM_KERNEL_AUTO, LBM_KERNEL_GENERIC, LBM_KERNEL_ROW, LBM_KERNEL_VEC, LBM_KERNEL_
 LBM_KERNEL_LDS = 6
 # lbm_config.optimize: arithmetic form of the collision.  The zero value is the
 # reference's own arithmetic (kernels.cl as ROCm's OpenCL compiler evaluates it;
-# 16-bit storage always computes in the fused form); EXACT is the CPU oracle's.
+# 16-bit storage always computes in the deviation form); EXACT is the CPU oracle's.
 LBM_ARITH_REFERENCE = LBM_ARITH_OPENCL = 0
 LBM_ARITH_FUSED, LBM_ARITH_EXACT = 1, 2
 ARITH_NAMES = {"reference": LBM_ARITH_REFERENCE, "opencl": LBM_ARITH_OPENCL, "fused": LBM_ARITH_FUSED,

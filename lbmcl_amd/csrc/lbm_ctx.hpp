@@ -414,7 +414,7 @@ inline int plan_from_config(const lbm_config& k, StepPlan* p) {
         return fail(LBM_ERR_INVALID,
                     "optimize must be LBM_ARITH_REFERENCE (LBM_ARITH_OPENCL), LBM_ARITH_FUSED or LBM_ARITH_EXACT");
     if (k.optimize == LBM_ARITH_EXACT && k.storage != LBM_STORE_NATIVE)
-        return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the fused form)");
+        return fail(LBM_ERR_INVALID, "LBM_ARITH_EXACT needs native storage (16-bit storage uses the deviation form)");
     if (k.kernel != LBM_KERNEL_AUTO && k.kernel != LBM_KERNEL_GENERIC && k.kernel != LBM_KERNEL_ROW &&
         k.kernel != LBM_KERNEL_VEC && k.kernel != LBM_KERNEL_VEC8 && k.kernel != LBM_KERNEL_TILE &&
         k.kernel != LBM_KERNEL_LDS)

@@ -211,7 +211,7 @@ class LBMCL {
     // Collision arithmetic (LBM_ARITH_*); call before setupSimulation.  By
     // default the constructor's `optimize` selects LBM_ARITH_FUSED, and
     // otherwise LBM_ARITH_OPENCL, the reference's own rounding on this GPU
-    // (16-bit storage always uses the fused form); LBM_ARITH_EXACT evaluates
+    // (16-bit storage always uses the deviation form); LBM_ARITH_EXACT evaluates
     // each operation as written.
     void setArithmetic(int a) { arith = a; }
 

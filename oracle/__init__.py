@@ -249,7 +249,7 @@ class PullSim:
     """The gather form (native, FP16- or BF16-shifted storage).
 
     fast selects the fused collision (or ARITH_CONTRACTED the contracted
-    expressions); the shifted storage forms always use the fused collision."""
+    expressions); the shifted storage forms always use the deviation form."""
 
     def __init__(self, dim, stride=None, viscosity=0.0089, velocity=0.05, precision="f32",
                  storage=STORE_NATIVE, threads=None, fast=False):
