@@ -1,7 +1,8 @@
 """The body force and the adjoint on the GPU.
 
 * the body force: a zero field is bit-identical to no force; a random field
-  matches the CPU oracle's forcing term bit for bit;
+  matches the CPU oracle's forcing term bit for bit, with native and with
+  16-bit storage;
 * the adjoint: J from the adjoint run equals J from a plain forward run;
   checkpointed and full-tape runs agree exactly; gradients match FP64 central
   finite differences (h = 1e-6 relative) on seeded random directions of
@@ -54,6 +55,30 @@ def test_force_matches_oracle(precision, kernel):
         assert nan_equal(b.read_rho(), o.rho) and nan_equal(b.read_u(), o.u)
         assert not nan_equal(b.read_u(), ref.read_u())
         ref.close()
+
+
+@pytest.mark.parametrize("storage,kernel,n,stride", [("fp16", capi.LBM_KERNEL_VEC8, 256, 4096),
+                                                      ("bf16", capi.LBM_KERNEL_VEC8, 256, 256),
+                                                      ("fp16", capi.LBM_KERNEL_TILE, 64, 8),
+                                                      ("bf16", capi.LBM_KERNEL_GENERIC, 32, 32)])
+def test_force_16bit_storage_matches_oracle(storage, kernel, n, stride):
+    """The body force with 16-bit storage: the kernels add the force term to the
+    scaled deviations (times the storage scale), the oracle to the unscaled
+    ones; the stored lattices and rho/u agree bit for bit."""
+    g = gaussian_random_field(n, 4.0, 1e-4, seed=7).astype(np.float32)
+    st = O.STORE_FP16 if storage == "fp16" else O.STORE_BF16
+    with capi.Context(n, stride=stride, storage=storage, kernel=kernel) as b:
+        o = O.PullSim(n, stride=stride, storage=st)
+        o.force = g.reshape(-1)
+        b.set_force(g)
+        b.initialize()
+        o.initialize()
+        for it in range(1, 6):
+            b.compute(it, update_macro=True)
+            o.step(True)
+        assert nan_equal(b.read_rho(), o.rho) and nan_equal(b.read_u(), o.u)
+        for buf, ob in ((0, o.A), (1, o.B)):
+            assert np.array_equal(b.read_state(buf).view(np.uint16), ob.view(np.uint16)), f"lattice {buf}"
 
 
 def _direction(n, rng):
