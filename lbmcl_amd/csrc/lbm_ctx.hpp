@@ -485,11 +485,26 @@ int launch_compute_opencl_f64(StepPlan* c, const void* src, void* dst, int updat
 // which stamps the kernel's own start and end into them (no separate marker
 // packets between launches, like the reference's profiling-queue events).
 template <class K, class A>
-inline void launch_on(StepPlan* c, K kernel, dim3 grid, dim3 block, hipEvent_t start, hipEvent_t stop, const A& a) {
+inline void launch_on(StepPlan* c, K kernel, dim3 grid, dim3 block, hipEvent_t start, hipEvent_t stop, const A& a,
+                      size_t shmem = 0) {
     if (start)
-        hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, start, stop, 0, a);
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, c->stream, start, stop, 0, a);
     else
-        hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, a);
+}
+
+// Occupancy cap of the vector kernel on lattices stored through the caches
+// (128^3): a few launch rounds of blocks per CU, where fewer, fuller rounds
+// beat the register-limited maximum -- measured at 128^3, FP32 4 waves per
+// SIMD instead of 5 (+1.2%), FP64 5 instead of 6 (+1.3%).  The cap is unused
+// dynamic LDS: 160 KiB / B - 2 KiB per block admits B blocks per CU, not B + 1,
+// for kernels with at most 2 KiB of static LDS (the vector kernels use
+// 0.6-1.5 KiB).
+inline size_t small_lattice_lds_pad(const StepPlan* c, unsigned block_threads) {
+    const unsigned waves_per_simd = c->tsize == 4 ? 4u : 5u;
+    const unsigned blocks_per_cu = waves_per_simd * 4u / (block_threads / 64u);
+    if (blocks_per_cu == 0) return 0;
+    return size_t(160 * 1024) / blocks_per_cu - 2048;
 }
 
 template <class T, int KIND, int ARITH>
@@ -565,7 +580,8 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
             if (kk == K_VEC)
                 launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 16, ARITH>, dim3(unsigned(nb)), block, e0, e1, a);
             else if (cached_stores(c))
-                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH, true>, dim3(unsigned(nb)), block, e0, e1, a);
+                launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH, true>, dim3(unsigned(nb)), block, e0, e1, a,
+                          small_lattice_lds_pad(c, block.x));
             else
                 launch_on(c, lbm::lbm_collide_stream_vec<T, KIND, 8, ARITH, false>, dim3(unsigned(nb)), block, e0, e1, a);
         } else {
