@@ -218,7 +218,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))

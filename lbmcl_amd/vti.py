@@ -9,8 +9,8 @@ Reads the two encodings the reference workflow meets:
   (target_results/*/ldc.0.*.vti): ``format="appended"``, base64 encoding,
   ``vtkZLibDataCompressor``, ``header_type="UInt32"``.
 
-The reference reads both with pyvista (verify.py:44-45), which is not part of
-this build; this module replaces it for verify.py's purpose.
+The reference reads both with pyvista (verify.py:44-45), which this package
+does not depend on; this module replaces it for verify.py's purpose.
 """
 from __future__ import annotations
 
