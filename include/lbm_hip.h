@@ -117,7 +117,8 @@ typedef struct lbm_config {
     int32_t optimize;    /* LBM_ARITH_*: 0 the reference's arithmetic, 1 is -o (fused collision) */
     int32_t kernel;      /* LBM_KERNEL_* */
     int32_t profile;     /* nonzero: record a start/stop event pair around every enqueue */
-    int32_t xcd_remap;   /* nonzero: map consecutive row blocks to one XCD (row kernels) */
+    int32_t xcd_remap;   /* 0 (default) or positive: consecutive row blocks go to one XCD (row and
+                            vector kernels); negative: the plain block order */
     int32_t raw_parameters; /* nonzero: use viscosity and velocity as given, without the 6-digit
                                rounding of the reference's -D literals (for gradient checks) */
 } lbm_config;

@@ -210,7 +210,7 @@ class Context:
         cfg.optimize = ARITH_NAMES[arith] if isinstance(arith, str) else int(arith)
         cfg.kernel = kernel
         cfg.profile = int(bool(profile))
-        cfg.xcd_remap = int(bool(xcd_remap))
+        cfg.xcd_remap = 1 if xcd_remap else -1  # 0 (a zeroed config) also means on
         cfg.raw_parameters = int(bool(raw_parameters))
         self.cfg = cfg
         self.stride = cfg.stride

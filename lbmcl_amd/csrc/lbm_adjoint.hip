@@ -171,7 +171,7 @@ int run_adjoint(lbm_ctx* c, const lbm_adjoint_config& cfg, lbm_adjoint_result* r
     a.U = T(c->U);
     a.objective = cfg.objective;
     a.plane_z = cfg.plane_z;
-    a.xcd_remap = c->cfg.xcd_remap;
+    a.xcd_remap = c->cfg.xcd_remap >= 0 ? 1 : 0;
     {
         // number of FLUID/MOVING cells in the plane z = plane_z (all interior cells of that plane but the
         // boundary-layer ones); counted on the host

@@ -522,7 +522,7 @@ inline int launch_compute_impl(StepPlan* c, const void* src, void* dst, int upda
     a.log2n = c->log2n;
     a.update_macro = update_macro;
     a.diagnostics = diag;
-    a.xcd_remap = c->cfg.xcd_remap;
+    a.xcd_remap = c->cfg.xcd_remap >= 0 ? 1 : 0;
     a.k.inv_tau = T(c->inv_tau);
     a.k.U = T(c->U);
     a.k.om1 = T(1) - T(c->inv_tau);
